@@ -139,14 +139,14 @@ int ome_abi_version(void);
 /* Mirrors OMEinsum.allow_loops (src/loop_einsum.jl:63-73) and backend options. */
 int ome_set_option(ome_ctx* ctx, int key, int64_t value);
 int ome_get_option(ome_ctx* ctx, int key, int64_t* value);
-int ome_device_info(ome_ctx* ctx, ome_device_info* info);
+int ome_get_device_info(ome_ctx* ctx, ome_device_info* info);
 int ome_synchronize(ome_ctx* ctx);
 
 /* ---- memory pool (replaces get_output_array / similar / unsafe_free!:
  *      src/loop_einsum.jl:48-61, ext/AMDGPUExt.jl:27-40,124-126) ------------------------ */
 int ome_alloc(ome_ctx* ctx, size_t bytes, void** ptr);
 int ome_free(ome_ctx* ctx, void* ptr);
-int ome_pool_stats(ome_ctx* ctx, ome_pool_stats* stats);
+int ome_get_pool_stats(ome_ctx* ctx, ome_pool_stats* stats);
 int ome_memcpy_h2d(ome_ctx* ctx, void* dst, const void* src, size_t bytes);
 int ome_memcpy_d2h(ome_ctx* ctx, void* dst, const void* src, size_t bytes);
 int ome_memcpy_d2d(ome_ctx* ctx, void* dst, const void* src, size_t bytes);
@@ -169,6 +169,11 @@ int ome_loop_einsum(ome_ctx* ctx, int dtype, int nin, const void* const* xs,
                     const int32_t* nix, const int32_t* ixs, void* y, int niy,
                     const int32_t* iy, int nlabels, const int64_t* sizes,
                     const ome_scalar* sx, const ome_scalar* sy);
+
+/* @flatten_addmul! y .= sy*y .+ sx*x over n elements (src/utils.jl:1-37); conj_x != 0 reads
+ * conj(x) (the conj(dy) of einsum_grad, src/autodiff.jl:24-30).  x == y is allowed. */
+int ome_addmul(ome_ctx* ctx, int dtype, int64_t n, const void* x, void* y,
+               const ome_scalar* sx, const ome_scalar* sy, int conj_x);
 
 /* tensorpermute!(C, A, perm, sx, sy): src/utils.jl:122-156.  dims are A's extents,
  * perm is 0-based (C dim p = A dim perm[p]). */
@@ -204,7 +209,7 @@ int ome_grouped_gemm(ome_ctx* ctx, int dtype, int ngroups, const int64_t* dims,
 int ome_tree_create(ome_ctx* ctx, int dtype, const int32_t* desc, int64_t desc_len,
                     int nlabels, const int64_t* sizes, int nslice,
                     const int32_t* slice_labels, ome_tree** out);
-int ome_tree_info(ome_tree* tree, ome_tree_info* info);
+int ome_get_tree_info(ome_tree* tree, ome_tree_info* info);
 /* einsum!(ne, xs, y, sx, sy, size_dict): src/einsequence.jl:270-287, src/slicing.jl:112-121.
  * For a sliced tree the cross-slice sum runs in FP64 (complex FP64 for C32) and is
  * rounded to the output type once. */

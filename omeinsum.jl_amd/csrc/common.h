@@ -1,0 +1,227 @@
+// Shared definitions of the MI355X contraction backend: element types, semiring
+// arithmetic, fast integer division for index decomposition, and the kernel descriptors.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+namespace ome {
+
+enum DType : int { F32 = 0, F64 = 1, C32 = 2, C64 = 3, I64 = 4, TF32 = 5, TF64 = 6 };
+
+inline size_t elem_size(int dt) {
+  switch (dt) {
+    case F32: case TF32: return 4;
+    case F64: case I64: case TF64: case C32: return 8;
+    case C64: return 16;
+  }
+  return 0;
+}
+inline bool is_complex(int dt) { return dt == C32 || dt == C64; }
+inline bool is_tropical(int dt) { return dt == TF32 || dt == TF64; }
+
+struct Scal {  // layout of ome_scalar
+  double re, im;
+  int64_t i;
+};
+
+struct c32 { float x, y; };
+struct c64 { double x, y; };
+
+// ---- semiring arithmetic, one struct per element type ----------------------------------
+// mul/add are the semiring's product and sum; zero/one its neutral elements.
+
+struct OpsF32 {
+  using T = float;
+  static constexpr int DT = F32;
+  __host__ __device__ static T zero() { return 0.f; }
+  __host__ __device__ static T one() { return 1.f; }
+  __host__ __device__ static T mul(T a, T b) { return a * b; }
+  __host__ __device__ static T add(T a, T b) { return a + b; }
+  __host__ __device__ static T fma(T a, T b, T c) { return a * b + c; }
+  __host__ __device__ static T from(const Scal& s) { return (float)s.re; }
+  static bool is_zero(const Scal& s) { return (float)s.re == 0.f; }
+};
+struct OpsF64 {
+  using T = double;
+  static constexpr int DT = F64;
+  __host__ __device__ static T zero() { return 0.0; }
+  __host__ __device__ static T one() { return 1.0; }
+  __host__ __device__ static T mul(T a, T b) { return a * b; }
+  __host__ __device__ static T add(T a, T b) { return a + b; }
+  __host__ __device__ static T fma(T a, T b, T c) { return a * b + c; }
+  __host__ __device__ static T from(const Scal& s) { return s.re; }
+  static bool is_zero(const Scal& s) { return s.re == 0.0; }
+};
+struct OpsC32 {
+  using T = c32;
+  static constexpr int DT = C32;
+  __host__ __device__ static T zero() { return {0.f, 0.f}; }
+  __host__ __device__ static T one() { return {1.f, 0.f}; }
+  __host__ __device__ static T mul(T a, T b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+  }
+  __host__ __device__ static T add(T a, T b) { return {a.x + b.x, a.y + b.y}; }
+  __host__ __device__ static T fma(T a, T b, T c) {
+    return {c.x + (a.x * b.x - a.y * b.y), c.y + (a.x * b.y + a.y * b.x)};
+  }
+  __host__ __device__ static T from(const Scal& s) { return {(float)s.re, (float)s.im}; }
+  static bool is_zero(const Scal& s) { return (float)s.re == 0.f && (float)s.im == 0.f; }
+};
+struct OpsC64 {
+  using T = c64;
+  static constexpr int DT = C64;
+  __host__ __device__ static T zero() { return {0.0, 0.0}; }
+  __host__ __device__ static T one() { return {1.0, 0.0}; }
+  __host__ __device__ static T mul(T a, T b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+  }
+  __host__ __device__ static T add(T a, T b) { return {a.x + b.x, a.y + b.y}; }
+  __host__ __device__ static T fma(T a, T b, T c) {
+    return {c.x + (a.x * b.x - a.y * b.y), c.y + (a.x * b.y + a.y * b.x)};
+  }
+  __host__ __device__ static T from(const Scal& s) { return {s.re, s.im}; }
+  static bool is_zero(const Scal& s) { return s.re == 0.0 && s.im == 0.0; }
+};
+// Int64: two's-complement wrapping arithmetic (Julia Int64), computed on uint64 so that
+// overflow is defined.
+struct OpsI64 {
+  using T = int64_t;
+  static constexpr int DT = I64;
+  __host__ __device__ static T zero() { return 0; }
+  __host__ __device__ static T one() { return 1; }
+  __host__ __device__ static T mul(T a, T b) { return (T)((uint64_t)a * (uint64_t)b); }
+  __host__ __device__ static T add(T a, T b) { return (T)((uint64_t)a + (uint64_t)b); }
+  __host__ __device__ static T fma(T a, T b, T c) { return add(mul(a, b), c); }
+  __host__ __device__ static T from(const Scal& s) { return s.i; }
+  static bool is_zero(const Scal& s) { return s.i == 0; }
+};
+// Tropical (max-plus): a (+) b = max(a, b), a (*) b = a + b, zero = -Inf, one = 0.
+struct OpsTF32 {
+  using T = float;
+  static constexpr int DT = TF32;
+  __host__ __device__ static T zero() { return -INFINITY; }
+  __host__ __device__ static T one() { return 0.f; }
+  __host__ __device__ static T mul(T a, T b) { return a + b; }
+  __host__ __device__ static T add(T a, T b) { return fmaxf(a, b); }
+  __host__ __device__ static T fma(T a, T b, T c) { return fmaxf(a + b, c); }
+  __host__ __device__ static T from(const Scal& s) { return (float)s.re; }
+  static bool is_zero(const Scal& s) { return std::isinf(s.re) && s.re < 0; }
+};
+struct OpsTF64 {
+  using T = double;
+  static constexpr int DT = TF64;
+  __host__ __device__ static T zero() { return -INFINITY; }
+  __host__ __device__ static T one() { return 0.0; }
+  __host__ __device__ static T mul(T a, T b) { return a + b; }
+  __host__ __device__ static T add(T a, T b) { return fmax(a, b); }
+  __host__ __device__ static T fma(T a, T b, T c) { return fmax(a + b, c); }
+  __host__ __device__ static T from(const Scal& s) { return s.re; }
+  static bool is_zero(const Scal& s) { return std::isinf(s.re) && s.re < 0; }
+};
+
+// Dispatch a functor templated on the Ops struct by runtime dtype.
+#define OME_DISPATCH(dt, OPS, ...)                          \
+  switch (dt) {                                             \
+    case F32: { using OPS = OpsF32; __VA_ARGS__; } break;   \
+    case F64: { using OPS = OpsF64; __VA_ARGS__; } break;   \
+    case C32: { using OPS = OpsC32; __VA_ARGS__; } break;   \
+    case C64: { using OPS = OpsC64; __VA_ARGS__; } break;   \
+    case I64: { using OPS = OpsI64; __VA_ARGS__; } break;   \
+    case TF32: { using OPS = OpsTF32; __VA_ARGS__; } break; \
+    case TF64: { using OPS = OpsTF64; __VA_ARGS__; } break; \
+    default: break;                                         \
+  }
+
+// Epilogue mode:  y <- sx*v (sy zero: y not read) | sy*y + sx*v | sy*y (sx zero).
+enum EpiMode : int { EPI_OVERWRITE = 0, EPI_AXPBY = 1, EPI_SCALE_ONLY = 2 };
+
+template <class O>
+__device__ __forceinline__ typename O::T epilogue(int mode, typename O::T sx, typename O::T sy,
+                                                  typename O::T v, const typename O::T* yp) {
+  if (mode == EPI_OVERWRITE) return O::mul(sx, v);
+  typename O::T y = *yp;
+  if (mode == EPI_SCALE_ONLY) return O::mul(sy, y);
+  return O::add(O::mul(sy, y), O::mul(sx, v));
+}
+
+template <class O>
+inline int epi_mode(const Scal& sx, const Scal& sy) {
+  if (O::is_zero(sx)) return EPI_SCALE_ONLY;
+  if (O::is_zero(sy)) return EPI_OVERWRITE;
+  return EPI_AXPBY;
+}
+
+// ---- fast unsigned division by an invariant divisor (Granlund & Montgomery 1994) -------
+// q = (umulhi(n, m) + n) >> s, valid for all 32-bit n with the 33-bit intermediate.
+struct FDiv {
+  uint32_t d, m, s;
+};
+inline FDiv make_fdiv(uint32_t d) {
+  FDiv f;
+  f.d = d ? d : 1;
+  uint32_t l = 0;
+  while ((1ull << l) < f.d) ++l;
+  f.s = l;
+  f.m = (uint32_t)((((1ull << l) - f.d) << 32) / f.d + 1);
+  if (f.d == 1) { f.m = 0; f.s = 0; }
+  return f;
+}
+__device__ __forceinline__ uint32_t fdiv_q(uint32_t n, const FDiv& f) {
+  return (uint32_t)(((uint64_t)__umulhi(n, f.m) + n) >> f.s);
+}
+
+constexpr int MAXR = 40;        // dims per permute after grouping
+constexpr int WALK_MAXIN = 16;  // inputs of one index walk
+constexpr int WALK_MAXL = 40;   // output dims / outer labels / inner labels
+constexpr int WALK_MAXS = 192;  // stride table entries (labels x inputs)
+
+struct PermDesc {
+  const void* x;
+  void* y;
+  int64_t n;          // elements
+  int rank;           // output dims
+  FDiv odiv[MAXR];    // output extents, first dim fastest
+  int64_t istride[MAXR];  // input stride of each output dim
+  Scal sx, sy;
+  int mode;
+  int conj;
+};
+
+struct WalkDesc {
+  const void* x[WALK_MAXIN];
+  void* y;
+  void* partial;      // [nsplit][ny] accumulators when the inner space is split
+  int nin;
+  int ny_dims;        // dims of y (repeats allowed)
+  int nouter;         // unique output labels
+  int ninner;
+  int64_t ny;         // elements of y
+  int64_t ninner_pts; // points of the inner space
+  int64_t chunk;      // inner points per split
+  int nsplit;
+  int mode;
+  Scal sx, sy;
+  FDiv ydiv[WALK_MAXL];
+  int8_t yslot[WALK_MAXL];    // y dim -> outer slot
+  FDiv idiv[WALK_MAXL];       // inner label extents (first fastest)
+  int32_t stride[WALK_MAXS];  // [label slot][input]; outer slots first, then inner
+};
+
+struct GemmDesc {
+  const void* a;
+  const void* b;
+  void* y;
+  int64_t M, N, K, B;
+  int64_t sam, sak, sab;
+  int64_t sbk, sbn, sbb;
+  int64_t sym, syn, syb;
+  Scal sx, sy;
+  int mode;
+  int ksplit;       // >1: partial products into work[ksplit][B][M][N], reduced after
+  void* work;
+};
+
+}  // namespace ome

@@ -1,0 +1,170 @@
+// Semiring GEMM on the vector ALUs: exact Int64 (wrapping) and tropical (max-plus)
+// products stay off the matrix cores, and small or oddly-strided float products use it too.
+//   y[m,n,b] = epilogue(sx, sy, sum_k a[m,k,b] * b[k,n,b], y[m,n,b])
+// with arbitrary element strides on every operand.
+#include "kernels.h"
+
+namespace ome {
+
+static inline int grid_cap(int64_t n, int per = 256) {
+  int64_t g = (n + per - 1) / per;
+  return (int)(g < 1 ? 1 : (g > 1048576 ? 1048576 : g));
+}
+
+// 64x64 output tile, BK=16, 256 threads each owning a 4x4 sub-tile (rows tm+16i, cols tn+16j).
+template <class O>
+__global__ __launch_bounds__(256) void k_gemm_tile(GemmDesc d, int64_t tiles_m, int64_t tiles_n,
+                                                   int64_t kchunk) {
+  using T = typename O::T;
+  constexpr int BM = 64, BN = 64, BK = 16;
+  __shared__ T As[BK][BM + 1];
+  __shared__ T Bs[BK][BN + 1];
+  const T* __restrict__ A = (const T*)d.a;
+  const T* __restrict__ B = (const T*)d.b;
+  int64_t bid = blockIdx.x;
+  const int64_t tm_i = bid % tiles_m; bid /= tiles_m;
+  const int64_t tn_i = bid % tiles_n; bid /= tiles_n;
+  const int64_t batch = bid % d.B; bid /= d.B;
+  const int64_t ks = bid;  // split-K index
+  const int64_t m0 = tm_i * BM, n0 = tn_i * BN;
+  const int64_t kbeg = ks * kchunk;
+  int64_t kend = kbeg + kchunk;
+  if (kend > d.K) kend = d.K;
+  const int t = threadIdx.x, tm = t & 15, tn = t >> 4;
+  const bool a_mfast = (d.sam == 1) || (d.K == 1);
+  const bool b_kfast = (d.sbk == 1) || (d.N == 1);
+  const T* Ab = A + batch * d.sab;
+  const T* Bb = B + batch * d.sbb;
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = O::zero();
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int e = t + 256 * q;
+      int mm, kk;
+      if (a_mfast) { mm = e & 63; kk = e >> 6; } else { kk = e & 15; mm = e >> 4; }
+      int64_t gm = m0 + mm, gk = k0 + kk;
+      As[kk][mm] = (gm < d.M && gk < kend) ? Ab[gm * d.sam + gk * d.sak] : O::zero();
+      int nn, kb;
+      if (b_kfast) { kb = e & 15; nn = e >> 4; } else { nn = e & 63; kb = e >> 6; }
+      int64_t gn = n0 + nn, gkb = k0 + kb;
+      Bs[kb][nn] = (gn < d.N && gkb < kend) ? Bb[gkb * d.sbk + gn * d.sbn] : O::zero();
+    }
+    __syncthreads();
+    const int kn = (int)((kend - k0) < BK ? (kend - k0) : BK);
+    for (int kk = 0; kk < kn; ++kk) {
+      T a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][tm + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tn + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = O::fma(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+  if (d.ksplit > 1) {
+    T* W = (T*)d.work + ((ks * d.B + batch) * d.M) * d.N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int64_t m = m0 + tm + 16 * i, n = n0 + tn + 16 * j;
+        if (m < d.M && n < d.N) W[m + n * d.M] = acc[i][j];
+      }
+    return;
+  }
+  const T sx = O::from(d.sx), sy = O::from(d.sy);
+  T* Y = (T*)d.y + batch * d.syb;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int64_t m = m0 + tm + 16 * i, n = n0 + tn + 16 * j;
+      if (m < d.M && n < d.N) {
+        T* yp = Y + m * d.sym + n * d.syn;
+        *yp = epilogue<O>(d.mode, sx, sy, acc[i][j], yp);
+      }
+    }
+}
+
+// Sum the ksplit partial products in split order and apply the epilogue.
+template <class O>
+__global__ __launch_bounds__(256) void k_gemm_splitk_reduce(GemmDesc d) {
+  using T = typename O::T;
+  const T sx = O::from(d.sx), sy = O::from(d.sy);
+  const int64_t per = d.M * d.N, total = per * d.B;
+  const T* W = (const T*)d.work;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t batch = e / per, r = e - batch * per;
+    const int64_t n = r / d.M, m = r - n * d.M;
+    T acc = O::zero();
+    for (int s = 0; s < d.ksplit; ++s) acc = O::add(acc, W[s * total + e]);
+    T* yp = (T*)d.y + batch * d.syb + m * d.sym + n * d.syn;
+    *yp = epilogue<O>(d.mode, sx, sy, acc, yp);
+  }
+}
+
+// One thread per output element, K walked directly from global memory: for tiny M/N or K
+// (tensor-network steps over dimension-2 labels) where a 64x64 tile would idle.
+template <class O>
+__global__ __launch_bounds__(256) void k_gemm_direct(GemmDesc d, int order_m_fast) {
+  using T = typename O::T;
+  const T sx = O::from(d.sx), sy = O::from(d.sy);
+  const T* __restrict__ A = (const T*)d.a;
+  const T* __restrict__ B = (const T*)d.b;
+  const int64_t per = d.M * d.N, total = per * d.B;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t batch = e / per, r = e - batch * per;
+    int64_t m, n;
+    if (order_m_fast) { n = r / d.M; m = r - n * d.M; } else { m = r / d.N; n = r - m * d.N; }
+    const T* ap = A + batch * d.sab + m * d.sam;
+    const T* bp = B + batch * d.sbb + n * d.sbn;
+    T acc = O::zero();
+    for (int64_t k = 0; k < d.K; ++k) acc = O::fma(ap[k * d.sak], bp[k * d.sbk], acc);
+    T* yp = (T*)d.y + batch * d.syb + m * d.sym + n * d.syn;
+    *yp = epilogue<O>(d.mode, sx, sy, acc, yp);
+  }
+}
+
+int64_t gemm_valu_workspace(int dt, const GemmDesc& d, int ksplit) {
+  return ksplit > 1 ? (int64_t)ksplit * d.B * d.M * d.N * (int64_t)elem_size(dt) : 0;
+}
+
+void launch_gemm_valu(int dt, GemmDesc d, int kernel, int ksplit, hipStream_t st) {
+  const int64_t outputs = d.M * d.N * d.B;
+  if (outputs <= 0) return;
+  OME_DRY_RETURN(ksplit > 1 ? 2 : 1);
+  OME_DISPATCH(dt, O, {
+    d.mode = epi_mode<O>(d.sx, d.sy);
+    if (kernel == GEMM_VALU_DIRECT) {
+      d.ksplit = 1;
+      int order_m_fast = (d.sym == 1 || d.N == 1) ? 1 : 0;
+      hipLaunchKernelGGL(k_gemm_direct<O>, dim3(grid_cap(outputs)), dim3(256), 0, st, d,
+                         order_m_fast);
+    } else {
+      if (ksplit < 1) ksplit = 1;
+      int64_t kchunk = (d.K + ksplit - 1) / ksplit;
+      kchunk = ((kchunk + 15) / 16) * 16;
+      if (kchunk < 16) kchunk = 16;
+      ksplit = (int)((d.K + kchunk - 1) / kchunk);
+      if (ksplit < 1) ksplit = 1;
+      d.ksplit = ksplit;
+      const int64_t tiles_m = (d.M + 63) / 64, tiles_n = (d.N + 63) / 64;
+      const int64_t blocks = tiles_m * tiles_n * d.B * ksplit;
+      hipLaunchKernelGGL(k_gemm_tile<O>, dim3((uint32_t)blocks), dim3(256), 0, st, d, tiles_m,
+                         tiles_n, kchunk);
+      if (ksplit > 1)
+        hipLaunchKernelGGL(k_gemm_splitk_reduce<O>, dim3(grid_cap(outputs)), dim3(256), 0, st, d);
+    }
+  });
+}
+
+}  // namespace ome

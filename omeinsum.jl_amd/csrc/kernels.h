@@ -1,0 +1,41 @@
+// Host-side launchers of the device kernels (all enqueue on the given stream).
+#pragma once
+#include "common.h"
+
+namespace ome {
+
+// Dry-run mode (thread-local): launchers count the kernels they would enqueue and return
+// without launching.  Used to size a plan's scratch memory before it is allocated.
+struct LaunchMode {
+  bool dry = false;
+  int64_t count = 0;
+};
+LaunchMode& launch_mode();
+#define OME_DRY_RETURN(n)            \
+  if (launch_mode().dry) {           \
+    launch_mode().count += (n);      \
+    return;                          \
+  }
+
+enum Prec : int { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_FP64 = 2 };
+enum GemmKernel : int { GEMM_AUTO = -1, GEMM_VALU_TILE = 0, GEMM_VALU_DIRECT = 1, GEMM_MFMA = 2 };
+
+void launch_axpby(int dt, const void* x, void* y, int64_t n, const Scal& sx, const Scal& sy,
+                  int conj, hipStream_t st);
+void launch_fill(int dt, void* y, int64_t n, const Scal& v, hipStream_t st);
+void launch_acc_f64(int dt, const void* x, double* acc, int64_t n, hipStream_t st);
+void launch_from_f64(int dt, const double* acc, void* y, int64_t n, const Scal& sx,
+                     const Scal& sy, hipStream_t st);
+// x <- x / max|x|; logacc[0] += log(max|x|).  work needs 1 + 1024 doubles.
+void launch_normalize(int dt, void* x, int64_t n, double* work, double* logacc, hipStream_t st);
+void launch_permute(int dt, const PermDesc& d, hipStream_t st);
+void launch_walk(int dt, const WalkDesc& d, hipStream_t st);
+
+int64_t gemm_valu_workspace(int dt, const GemmDesc& d, int ksplit);
+void launch_gemm_valu(int dt, GemmDesc d, int kernel, int ksplit, hipStream_t st);
+
+int mfma_plan_ksplit(const GemmDesc& d, int requested);
+int64_t mfma_workspace(int dt, const GemmDesc& d, int ksplit);
+void launch_gemm_mfma(int prec, GemmDesc d, int ksplit, hipStream_t st);
+
+}  // namespace ome

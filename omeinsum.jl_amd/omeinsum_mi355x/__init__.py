@@ -14,13 +14,14 @@ from .contractionorder import (GreedyMethod, TreeSASlicer, optimize_code, slice_
                                peak_memory, contraction_complexity, writejson, readjson)
 
 _LAZY = {
-    "einsum": "einsum", "einsum_inplace": "einsum", "loop_einsum": "einsum",
-    "loop_einsum_inplace": "einsum", "allow_loops": "einsum", "tensorpermute": "einsum",
-    "batched_gemm": "einsum", "grouped_gemm": "einsum", "asarray": "device",
+    "einsum": "ein_ops", "einsum_inplace": "ein_ops", "loop_einsum": "ein_ops",
+    "loop_einsum_inplace": "ein_ops", "allow_loops": "ein_ops", "tensorpermute": "ein_ops",
+    "batched_gemm": "ein_ops", "grouped_gemm": "ein_ops", "asarray": "device",
     "asscalar": "device", "Tropical": "device", "to_device": "device", "to_numpy": "device",
     "EinsumBackend": "backends", "DefaultBackend": "backends", "MI355XBackend": "backends",
     "set_einsum_backend": "backends", "get_einsum_backend": "backends",
     "einsum_grad": "autodiff", "cost_and_gradient": "autodiff", "einsum_backward_rule": "autodiff",
+    "einsum_autograd": "autodiff",
     "Server": "server", "library": "_lib",
 }
 

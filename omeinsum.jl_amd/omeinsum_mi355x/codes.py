@@ -72,7 +72,7 @@ class EinCode:
         return ", ".join(j(ix) for ix in self.ixs) + " -> " + j(self.iy)
 
     def __call__(self, *xs, size_info=None, **kw):
-        from . import einsum as _e
+        from . import ein_ops as _e
         size_dict = get_size_dict(self.ixs, xs, size_info)
         return _e.einsum(self, xs, size_dict, **kw)
 
@@ -334,7 +334,7 @@ class NestedEinsum:
         return "(" + ", ".join(repr(a) for a in self.args) + " | " + repr(self.eins) + ")"
 
     def __call__(self, *xs, size_info=None, **kw):
-        from . import einsum as _e
+        from . import ein_ops as _e
         size_dict = get_size_dict(self.getixsv(), xs, size_info)
         return _e.einsum(self, xs, size_dict, **kw)
 
@@ -379,7 +379,7 @@ class SlicedEinsum:
         return "SlicedEinsum(%r, %r)" % (self.slicing, self.eins)
 
     def __call__(self, *xs, size_info=None, **kw):
-        from . import einsum as _e
+        from . import ein_ops as _e
         size_dict = get_size_dict(self.getixsv(), xs, size_info)
         return _e.einsum(self, xs, size_dict, **kw)
 

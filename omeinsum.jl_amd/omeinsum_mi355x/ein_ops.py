@@ -1,0 +1,202 @@
+"""einsum / einsum! / loop_einsum front end (mirrors src/einsum.jl:24-50,
+src/loop_einsum.jl:8-32, src/utils.jl:122-177) over the C ABI.
+
+Every contraction is enqueued on the GPU by libomeinsum_mi355x.so on torch's current stream.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .codes import EinCode, NestedEinsum, SlicedEinsum, get_size_dict
+from .device import Tropical, dtype_code, scalar, to_device
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _unwrap(xs):
+    """-> (device tensors, tropical flag, common torch dtype)."""
+    torch = _torch()
+    trop = [isinstance(x, Tropical) for x in xs]
+    if any(trop) and not all(trop):
+        raise TypeError("cannot mix Tropical and ordinary tensors in one contraction")
+    ts = [to_device(x.data if isinstance(x, Tropical) else x) for x in xs]
+    T = ts[0].dtype
+    for t in ts[1:]:
+        T = torch.promote_types(T, t.dtype)
+    # promote_type of get_output_array (src/loop_einsum.jl:48-61); align_eltypes converts
+    ts = [t if t.dtype == T else t.to(T) for t in ts]
+    return ts, all(trop) and len(trop) > 0, T
+
+
+def _labels(ixs, iy):
+    ids = {}
+    for ix in list(ixs) + [iy]:
+        for l in ix:
+            if l not in ids:
+                ids[l] = len(ids)
+    return ids
+
+
+def _call_flat(ixs, iy, ts, yt, code, sx, sy, size_dict, force_loop=False):
+    """One flat einsum on the device.  Row-major tensors are passed with reversed label
+    lists (see device.py)."""
+    ctx = _lib.context(yt.device.index)
+    ids = _labels(ixs, iy)
+    sizes = (ctypes.c_int64 * max(len(ids), 1))()
+    for l, i in ids.items():
+        if l not in size_dict:
+            raise KeyError("size of label %r is unknown (pass size_info)" % (l,))
+        sizes[i] = int(size_dict[l])
+    n = len(ixs)
+    nix = (ctypes.c_int32 * n)(*[len(ix) for ix in ixs])
+    flat = [ids[l] for ix in ixs for l in reversed(list(ix))]
+    cixs = (ctypes.c_int32 * max(len(flat), 1))(*flat)
+    ciy = (ctypes.c_int32 * max(len(iy), 1))(*[ids[l] for l in reversed(list(iy))])
+    ptrs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts])
+    s_x, s_y = scalar(sx, code), scalar(sy, code)
+    fn = ctx.lib.ome_loop_einsum if force_loop else ctx.lib.ome_einsum
+    _lib.check(fn(ctx.h, code, n, ptrs, nix, cixs, ctypes.c_void_p(yt.data_ptr()), len(iy), ciy,
+                  len(ids), sizes, ctypes.byref(s_x), ctypes.byref(s_y)), ctx.h)
+
+
+def _flat_einsum_inplace(ixs, iy, xs, y, sx, sy, size_dict, force_loop=False):
+    ts, trop, T = _unwrap(xs)
+    yt = y.data if isinstance(y, Tropical) else y
+    if trop != isinstance(y, Tropical):
+        raise TypeError("output and inputs must both be Tropical or both ordinary")
+    if yt.dtype != T:
+        ts = [t.to(yt.dtype) for t in ts]
+    if not yt.is_contiguous():
+        raise ValueError("output tensor must be contiguous")
+    for ix, t in zip(ixs, ts):
+        if t.dim() != len(ix):
+            from .codes import ArgumentError
+            raise ArgumentError("indices %r invalid for tensor with ndims = %d" % (tuple(ix), t.dim()))
+    code = dtype_code(yt.dtype, trop)
+    _call_flat(ixs, iy, ts, yt, code, sx, sy, size_dict, force_loop)
+    return y
+
+
+def _new_output(xs, shape):
+    ts, trop, T = _unwrap(xs)
+    torch = _torch()
+    y = torch.empty(tuple(int(s) for s in shape), dtype=T, device=ts[0].device)
+    return Tropical(y) if trop else y
+
+
+def flat_einsum(code, xs, size_dict):
+    """einsum(code::EinCode, xs, size_dict) (src/einsum.jl:24-27)."""
+    y = _new_output(xs, [size_dict[l] for l in code.iy])
+    return _flat_einsum_inplace(code.ixs, code.iy, xs, y, True, False, size_dict)
+
+
+def einsum(code, xs, size_dict=None, **kw):
+    """einsum(code, xs, size_dict): EinCode, NestedEinsum or SlicedEinsum, dispatched on the
+    active backend (src/backends.jl:108,219)."""
+    from .backends import get_einsum_backend
+    xs = tuple(xs)
+    if size_dict is None:
+        size_dict = get_size_dict(code.getixsv(), xs)
+    return get_einsum_backend().einsum(code, xs, size_dict, **kw)
+
+
+def einsum_inplace(code, xs, y, sx=True, sy=False, size_dict=None):
+    """einsum!(code, xs, y, sx, sy, size_dict) (src/einsum.jl:43-50, src/einsequence.jl:270-277,
+    src/slicing.jl:112-121)."""
+    from .backends import get_einsum_backend
+    xs = tuple(xs)
+    if size_dict is None:
+        size_dict = get_size_dict(code.getixsv(), xs)
+    return get_einsum_backend().einsum_inplace(code, xs, y, sx, sy, size_dict)
+
+
+def loop_einsum(code, xs, size_dict=None):
+    """loop_einsum(code, xs, size_dict) (src/loop_einsum.jl:8-13): the index walk."""
+    xs = tuple(xs)
+    if size_dict is None:
+        size_dict = get_size_dict(code.ixs, xs)
+    y = _new_output(xs, [size_dict[l] for l in code.iy])
+    return _flat_einsum_inplace(code.ixs, code.iy, xs, y, True, False, size_dict, force_loop=True)
+
+
+def loop_einsum_inplace(ixs, iy, xs, y, sx, sy, size_dict):
+    """loop_einsum!(ixs, iy, xs, y, sx, sy, size_dict) (src/loop_einsum.jl:21-32)."""
+    return _flat_einsum_inplace(ixs, iy, tuple(xs), y, sx, sy, size_dict, force_loop=True)
+
+
+def allow_loops(flag):
+    """allow_loops(flag) (src/loop_einsum.jl:63-73)."""
+    _lib.context().set_option(_lib.OPT_ALLOW_LOOPS, 1 if flag else 0)
+    return flag
+
+
+def tensorpermute(C, A, perm, sx=True, sy=False):
+    """tensorpermute!(C, A, perm, sx, sy) (src/utils.jl:122-156) with 0-based perm:
+    C = permutedims(A, perm) scaled."""
+    ts, trop, T = _unwrap([A])
+    a = ts[0]
+    ct = C.data if isinstance(C, Tropical) else C
+    ctx = _lib.context(ct.device.index)
+    code = dtype_code(ct.dtype, trop)
+    nd = a.dim()
+    # row-major axis p is column-major dim nd-1-p
+    dims = (ctypes.c_int64 * max(nd, 1))(*[a.shape[nd - 1 - q] for q in range(nd)])
+    cperm = (ctypes.c_int32 * max(nd, 1))(*[nd - 1 - perm[nd - 1 - q] for q in range(nd)])
+    s_x, s_y = scalar(sx, code), scalar(sy, code)
+    _lib.check(ctx.lib.ome_permute(ctx.h, code, nd, dims, cperm, ctypes.c_void_p(a.data_ptr()),
+                                   ctypes.c_void_p(ct.data_ptr()), ctypes.byref(s_x),
+                                   ctypes.byref(s_y)), ctx.h)
+    return C
+
+
+def batched_gemm(ta, tb, alpha, A, B, beta, C):
+    """_batched_gemm!(C1, C2, alpha, A, B, beta, C) (src/utils.jl:162-177) in torch's row-major
+    convention: A is (batch, m, k) ('N') or (batch, k, m) ('T'), B is (batch, k, n) or
+    (batch, n, k), C is (batch, m, n); C = alpha*op(A) op(B) + beta*C."""
+    ts, trop, T = _unwrap([A, B])
+    a, b = ts
+    ct = C.data if isinstance(C, Tropical) else C
+    ctx = _lib.context(ct.device.index)
+    code = dtype_code(ct.dtype, trop)
+    batch, m, n = ct.shape
+    k = a.shape[2] if ta == "N" else a.shape[1]
+    al, be = scalar(alpha, code), scalar(beta, code)
+    # row-major C(b,m,n) is column-major C^T (n,m,b) = op(B)^T op(A)^T
+    _lib.check(ctx.lib.ome_batched_gemm(ctx.h, code, tb.encode(), ta.encode(), n, m, k, batch,
+                                        ctypes.byref(al), ctypes.c_void_p(b.data_ptr()),
+                                        ctypes.c_void_p(a.data_ptr()), ctypes.byref(be),
+                                        ctypes.c_void_p(ct.data_ptr())), ctx.h)
+    return C
+
+
+def grouped_gemm(problems, alpha=True, beta=False):
+    """Many independent batched GEMMs of uneven shapes in one persistent launch.
+    problems: list of (A, B, C) row-major tensors with A (batch, m, k), B (batch, k, n),
+    C (batch, m, n) (torch convention).  C = alpha*A@B + beta*C for each problem."""
+    if not problems:
+        return
+    ts, trop, T = _unwrap([p[0] for p in problems] + [p[1] for p in problems])
+    na = len(problems)
+    ct0 = problems[0][2]
+    ctx = _lib.context(ct0.device.index)
+    code = dtype_code(ct0.dtype, trop)
+    dims = (ctypes.c_int64 * (4 * na))()
+    pa = (ctypes.c_void_p * na)()
+    pb = (ctypes.c_void_p * na)()
+    pc = (ctypes.c_void_p * na)()
+    for g, (A, B, C) in enumerate(problems):
+        a, b = ts[g], ts[na + g]
+        batch, m, k = a.shape
+        n = b.shape[2]
+        # column-major view: C^T (n, m, batch) = B^T (n, k) * A^T (k, m)
+        dims[4 * g:4 * g + 4] = [n, m, k, batch]
+        pa[g] = b.data_ptr()
+        pb[g] = a.data_ptr()
+        pc[g] = C.data_ptr()
+    al, be = scalar(alpha, code), scalar(beta, code)
+    _lib.check(ctx.lib.ome_grouped_gemm(ctx.h, code, na, dims, pa, pb, pc, ctypes.byref(al),
+                                        ctypes.byref(be)), ctx.h)
