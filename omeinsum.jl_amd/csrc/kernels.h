@@ -38,4 +38,10 @@ int mfma_plan_ksplit(const GemmDesc& d, int requested);
 int64_t mfma_workspace(int dt, const GemmDesc& d, int ksplit);
 void launch_gemm_mfma(int prec, GemmDesc d, int ksplit, hipStream_t st);
 
+// Packed BF16x3 GEMM (Float32, or ComplexF32 realified into one real GEMM).
+int64_t packed_bytes(int64_t rows, int64_t k, int64_t batch);
+bool packed_plain_overwrite(const GemmDesc& g);
+void launch_gemm_packed(int dt, const GemmDesc& g, void* pa, void* pb, float* work, int ksplit,
+                        hipStream_t st);
+
 }  // namespace ome

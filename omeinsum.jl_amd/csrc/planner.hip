@@ -222,9 +222,34 @@ static Status do_unary(ome_ctx* ctx, int dt, const void* x, const std::vector<in
 
 // ---- GEMM dispatch -------------------------------------------------------------------------
 
+// BF16x3 on packed operands: Float32 and ComplexF32 products large enough that the one-pass
+// pack (split + tile-major relayout) is amortised by the GEMM.
+static Status run_gemm_packed(ome_ctx* ctx, int dt, const GemmDesc& d, hipStream_t st) {
+  const bool cplx = dt == C32;
+  const int64_t M = d.M, N = cplx ? 2 * d.N : d.N, K = cplx ? 2 * d.K : d.K;
+  const int64_t tiles = ((M + 127) / 128) * ((N + 127) / 128) * d.B;
+  const int64_t ktiles = (K + 31) / 32;
+  int ks = 1;
+  while (tiles * ks < 512 && ktiles / (ks * 2) >= 8) ks *= 2;
+  Temp pa(ctx, (size_t)packed_bytes(M, K, d.B));
+  Temp pb(ctx, (size_t)packed_bytes(N, K, d.B));
+  const bool via_work = ks > 1 || (cplx && !packed_plain_overwrite(d));
+  Temp work(ctx, via_work ? (size_t)ks * d.B * M * N * 4 : 0);
+  if (!pa.p || !pb.p || (via_work && !work.p))
+    return Status::err(OME_ERR_OOM, "device pool exhausted (packed GEMM operands)");
+  launch_gemm_packed(dt, d, pa.p, pb.p, (float*)work.p, ks, st);
+  return Status::ok();
+}
+
 Status run_gemm(ome_ctx* ctx, int dt, GemmDesc d, hipStream_t st, int precision) {
   if (d.M <= 0 || d.N <= 0 || d.B <= 0) return Status::ok();
   int kernel = ctx->opt.gemm_kernel;
+  if (precision == PREC_BF16X3 && (dt == F32 || dt == C32) &&
+      (kernel == GEMM_AUTO || kernel == GEMM_MFMA)) {
+    const double flops = 2.0 * d.M * d.N * d.K * d.B * (dt == C32 ? 4 : 1);
+    if (d.M >= 64 && d.N >= 64 && d.K >= 64 && flops >= 3.0e7)
+      return run_gemm_packed(ctx, dt, d, st);
+  }
   const bool mfma_type = (dt == F32 || dt == F64);
   const bool a_unit = d.sam == 1 || d.sak == 1 || d.M == 1 || d.K == 1;
   const bool b_unit = d.sbk == 1 || d.sbn == 1 || d.N == 1 || d.K == 1;

@@ -358,3 +358,29 @@ def test_tropical_gemm_exact(dtype):
     got = run([["i", "j"], ["j", "k"]], ["i", "k"], [a, b], tropical=True)
     ref = np.max(a[:, :, None] + b[None, :, :], axis=1)
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
+@pytest.mark.parametrize("spec", ["ij,jk->ik", "ji,kj->ik", "ijb,jkb->ikb", "ajb,bkj->bka", "ij,jk->ki"])
+@pytest.mark.parametrize("scal", [(True, False), (1.5 - 0.5j, 0.25)])
+def test_packed_bf16x3(dtype, spec, scal):
+    # the packed BF16x3 path (large products); ComplexF32 is realified into one real GEMM
+    o = om()
+    ixs, iy = parse(spec)
+    sd = {"i": 300, "j": 520, "k": 260, "a": 129, "b": 2}
+    r = rng(1414)
+    xs = [rand(r, [sd[l] for l in ix], dtype) for ix in ixs]
+    sx, sy = scal
+    if dtype == np.float32 and not isinstance(sx, bool):
+        sx = 1.5
+    y0 = rand(r, [sd[l] for l in iy], dtype)
+    o.set_einsum_backend(o.MI355XBackend(precision="bf16x3"))
+    try:
+        got = run(ixs, iy, xs, sd, y0=y0, sx=sx, sy=sy)
+    finally:
+        o.set_einsum_backend(o.DefaultBackend())
+    W = wide(dtype)
+    ref = db.einsum(ixs, iy, [x.astype(W) for x in xs], sd)
+    if not isinstance(sx, bool):
+        ref = sx * ref + sy * y0.astype(W)
+    assert_close(got, ref, dtype, 16 * 520, spec)
