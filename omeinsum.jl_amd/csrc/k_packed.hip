@@ -10,17 +10,20 @@
 // 2. GEMM: 128x128 tiles, 4 waves of 64x64, K-steps of 32 with a register prefetch of the next
 //    step (plain 16-byte loads, no conversion in the loop); per 16-k sub-step every wave issues
 //    3 v_mfma_f32_32x32x16_bf16 per 32x32 block: ah*bh + ah*bl + al*bh, FP32 accumulate.
+#include <cstdlib>
+#include <cstring>
+
 #include "kernels.h"
 
 namespace ome {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int PK_ROWS = 128;
-constexpr int PK_K = 32;
-constexpr int PK_TILE = 2 * PK_ROWS * PK_K;  // bf16 elements per packed tile
-constexpr int PK_LDK = PK_K + 8;             // LDS row stride (bf16): 80 bytes
+constexpr int PK_K = 32;          // k per packed tile
+constexpr int PK_LDK = PK_K + 8;  // LDS row stride (bf16): 80 bytes, conflict-free b128 reads
+__host__ __device__ constexpr int pk_tile_elems(int rows) { return 2 * rows * PK_K; }
 
 __device__ __forceinline__ uint16_t f2bf_rn(float x) {
   uint32_t u = __float_as_uint(x);
@@ -66,35 +69,38 @@ __device__ __forceinline__ float pack_fetch(const PackDesc& d, const float* __re
   return nim ? p[0] : -p[1];
 }
 
+// TR rows per packed tile (128 or 256).
+template <int TR>
 __global__ __launch_bounds__(256) void k_pack_bf16x3(PackDesc d) {
-  __shared__ float tile[PK_ROWS][PK_K + 1];
+  __shared__ float tile[TR][PK_K + 1];
   int64_t bid = blockIdx.x;
   const int64_t tk = bid % d.tiles_k; bid /= d.tiles_k;
   const int64_t tr = bid % d.tiles_r; bid /= d.tiles_r;
   const int64_t batch = bid;
   const float* base = (const float*)d.src + batch * d.sb * (d.mode ? 2 : 1);
-  const int64_t r0 = tr * PK_ROWS, k0 = tk * PK_K;
+  const int64_t r0 = tr * TR, k0 = tk * PK_K;
   const int t = threadIdx.x;
   if (d.rfast) {
     // lanes along rows
 #pragma unroll 4
-    for (int i = 0; i < 16; ++i) {
-      const int r = t & 127, k = (t >> 7) + 2 * i;
+    for (int i = 0; i < TR / 8; ++i) {
+      const int e = t + 256 * i;
+      const int r = e % TR, k = e / TR;
       tile[r][k] = pack_fetch(d, base, r0 + r, k0 + k);
     }
   } else {
     // lanes along k
 #pragma unroll 4
-    for (int i = 0; i < 16; ++i) {
+    for (int i = 0; i < TR / 8; ++i) {
       const int k = t & 31, r = (t >> 5) + 8 * i;
       tile[r][k] = pack_fetch(d, base, r0 + r, k0 + k);
     }
   }
   __syncthreads();
-  uint16_t* out = d.dst + ((batch * d.tiles_r + tr) * d.tiles_k + tk) * (int64_t)PK_TILE;
-  // 512 chunks of 8 bf16 per plane; thread t emits chunks t and t + 256 of both planes
+  uint16_t* out = d.dst + ((batch * d.tiles_r + tr) * d.tiles_k + tk) * (int64_t)pk_tile_elems(TR);
+  // TR*4 chunks of 8 bf16 per plane; thread t emits chunks t + 256q of both planes
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < TR / 64; ++q) {
     const int c = t + 256 * q;
     const int r = c >> 2, kq = (c & 3) * 8;
     uint16_t h[8], l[8];
@@ -109,8 +115,10 @@ __global__ __launch_bounds__(256) void k_pack_bf16x3(PackDesc d) {
     vh.z = h[4] | ((uint32_t)h[5] << 16); vh.w = h[6] | ((uint32_t)h[7] << 16);
     vl.x = l[0] | ((uint32_t)l[1] << 16); vl.y = l[2] | ((uint32_t)l[3] << 16);
     vl.z = l[4] | ((uint32_t)l[5] << 16); vl.w = l[6] | ((uint32_t)l[7] << 16);
-    *(uint4*)(out + c * 8) = vh;
-    *(uint4*)(out + PK_ROWS * PK_K + c * 8) = vl;
+    // row r's 16-byte chunk j lives at slot j ^ ((r >> 2) & 3): conflict-free b128 reads
+    const int slot = r * 4 + ((c & 3) ^ ((r >> 2) & 3));
+    *(uint4*)(out + slot * 8) = vh;
+    *(uint4*)(out + TR * PK_K + slot * 8) = vl;
   }
 }
 
@@ -153,82 +161,136 @@ __device__ __forceinline__ void pk_map(int64_t bid, int64_t nblocks, const Packe
   tn = in / gsz;
 }
 
-__global__ __launch_bounds__(256, 2) void k_gemm_packed_bf16x3(PackedGemmDesc d) {
-  __shared__ __attribute__((aligned(16))) uint16_t sA[2][PK_ROWS][PK_LDK];
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][PK_ROWS][PK_LDK];
+// Workgroup tile TM x TN (4 waves, 2 x 2, each (TM/2) x (TN/2) = MB x NB blocks of 32x32).
+// Per K-step of 32:
+//   1. wait for this step's tiles (copied global -> LDS by global_load_lds_dwordx4, one 1 KiB
+//      block per instruction), barrier;
+//   2. every wave reads all of its fragments for the step (both 16-k halves, hi and lo planes);
+//   3. barrier, then issue the copies of the next step into the same LDS buffer;
+//   4. run the step's MFMAs from registers while those copies are in flight.
+// s_waitcnt immediate that waits until at most n vector-memory operations are outstanding
+// (vmcnt bits [3:0] and [15:14]; expcnt and lgkmcnt left at their no-wait maxima).
+constexpr int waitcnt_vm(int n) { return (n & 0xF) | (((n >> 4) & 3) << 14) | (0x7 << 4) | (0xF << 8); }
+
+// Workgroup barrier without the memory fence of __syncthreads (which would wait for every
+// outstanding copy).  Callers wait for what they need first; the empty asm statements keep
+// the compiler from moving LDS accesses across the barrier.
+__device__ __forceinline__ void raw_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+constexpr int WAIT_LGKM0 = 0xC07F;  // lgkmcnt(0): this wave's LDS reads have completed
+
+template <int TM, int TN, int STAGES>
+__global__ __launch_bounds__(256) void k_gemm_packed_bf16x3(PackedGemmDesc d) {
+  constexpr int MB = TM / 64, NB = TN / 64;
+  constexpr int A_BYTES = TM * PK_K * 2 * 2, B_BYTES = TN * PK_K * 2 * 2;
+  constexpr int STAGE = A_BYTES + B_BYTES;
+  constexpr int CHUNKS = STAGE / 1024;  // 1 KiB copies per K-step
+  constexpr int CPW = CHUNKS / 4;       // copies per wave per K-step
+  static_assert(CHUNKS % 4 == 0, "copies must split evenly over 4 waves");
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[STAGES * STAGE];
   int64_t tm, tn, batch, ks;
   pk_map(blockIdx.x, gridDim.x, d, tm, tn, batch, ks);
   const int t = threadIdx.x;
-  const int lane = t & 63, w = t >> 6;
-  const int wm = (w & 1) * 64, wn = (w >> 1) * 64;
+  // the wave index is uniform across the wave: say so, the LDS-copy destination must be
+  const int lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
   const int64_t kt0 = ks * d.kt_per_split;
   int64_t kt1 = kt0 + d.kt_per_split;
   if (kt1 > d.tiles_k) kt1 = d.tiles_k;
-  const uint4* pa = (const uint4*)(d.a + ((batch * d.tiles_m + tm) * d.tiles_k) * (int64_t)PK_TILE);
-  const uint4* pb = (const uint4*)(d.b + ((batch * d.tiles_n + tn) * d.tiles_k) * (int64_t)PK_TILE);
-  // chunk c (16 bytes) of a tile: plane c/512, row (c%512)/4, k (c%4)*8
-  uint4 ra[4], rb[4];
-  auto gload = [&](int64_t kt) {
+  const uint8_t* ga = (const uint8_t*)(d.a + ((batch * d.tiles_m + tm) * d.tiles_k) * (int64_t)pk_tile_elems(TM));
+  const uint8_t* gb = (const uint8_t*)(d.b + ((batch * d.tiles_n + tn) * d.tiles_k) * (int64_t)pk_tile_elems(TN));
+  // wave w copies blocks w*CPW .. +CPW of [A tile | B tile] of step kt into stage kt % STAGES
+  auto issue = [&](int64_t kt) __attribute__((always_inline)) {
+    const int sbase = (int)(kt % STAGES) * STAGE;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      ra[q] = pa[kt * (PK_TILE / 8) + t + 256 * q];
-      rb[q] = pb[kt * (PK_TILE / 8) + t + 256 * q];
+    for (int i = 0; i < CPW; ++i) {
+      const int c = w * CPW + i;
+      const uint8_t* src = (c * 1024 < A_BYTES) ? ga + kt * A_BYTES + c * 1024
+                                                 : gb + kt * B_BYTES + (c * 1024 - A_BYTES);
+      const int loff = __builtin_amdgcn_readfirstlane(sbase + c * 1024);  // M0: wave-uniform
+      __builtin_amdgcn_global_load_lds((const void*)(src + lane * 16),
+                                       (__attribute__((address_space(3))) void*)(lds + loff),
+                                       16, 0, 0);
     }
   };
-  auto lstore = [&]() {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int c = t + 256 * q;
-      const int p = c >> 9, r = (c & 511) >> 2, kq = (c & 3) * 8;
-      *(uint4*)&sA[p][r][kq] = ra[q];
-      *(uint4*)&sB[p][r][kq] = rb[q];
-    }
+  // fragment of row r (within its tile), 16-byte chunk j, plane p
+  auto frag = [&](int base, int rows, int p, int r, int j) __attribute__((always_inline)) {
+    const int off = base + p * rows * 64 + r * 64 + 16 * (j ^ ((r >> 2) & 3));
+    return *(const bf16x8_t*)(lds + off);
   };
-  floatx16 acc[2][2];
+  floatx16 acc[NB][MB];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < NB; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < MB; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
-  if (kt0 < kt1) gload(kt0);
+  // prologue: the first copies in flight (STAGES - 1 steps; one step when STAGES == 1)
+  constexpr int AHEAD = STAGES > 1 ? STAGES - 1 : 1;
+#pragma unroll
+  for (int s = 0; s < AHEAD; ++s)
+    if (kt0 + s < kt1) issue(kt0 + s);
+  const int jl = lane >> 5;  // k half of the 16-k MFMA operand (0: k 0-7, 1: k 8-15)
   for (int64_t kt = kt0; kt < kt1; ++kt) {
-    lstore();
-    __syncthreads();
-    if (kt + 1 < kt1) gload(kt + 1);
+    // with STAGES > 1 the copy of step kt+STAGES-1 goes out now, into the buffer of step
+    // kt-1 whose fragment reads all waves finished before the last barrier
+    if (STAGES > 1 && kt + STAGES - 1 < kt1) issue(kt + STAGES - 1);
+    // wait for this wave's copies of step kt; copies of later steps may stay in flight
+    const int64_t later = STAGES > 1 ? ((kt1 - 1 - kt) < STAGES - 1 ? (kt1 - 1 - kt) : STAGES - 1) : 0;
+    if (later >= 2) __builtin_amdgcn_s_waitcnt(waitcnt_vm(2 * CPW));
+    else if (later == 1) __builtin_amdgcn_s_waitcnt(waitcnt_vm(CPW));
+    else __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    raw_barrier();  // every wave's copies of step kt are in LDS
+    const int sbase = (int)(kt % STAGES) * STAGE;
+    bf16x8_t ah[2][NB], al[2][NB], bh[2][MB], bl[2][MB];
 #pragma unroll
-    for (int ksub = 0; ksub < PK_K; ksub += 16) {
-      const int kk = ksub + 8 * (lane >> 5);
-      bf16x8_t ah[2], al[2], bh[2], bl[2];
+    for (int h = 0; h < 2; ++h) {
 #pragma unroll
-      for (int nb = 0; nb < 2; ++nb) {
-        ah[nb] = *(const bf16x8_t*)&sB[0][wn + nb * 32 + (lane & 31)][kk];
-        al[nb] = *(const bf16x8_t*)&sB[1][wn + nb * 32 + (lane & 31)][kk];
+      for (int nb = 0; nb < NB; ++nb) {
+        const int r = wn + nb * 32 + (lane & 31);
+        ah[h][nb] = frag(sbase + A_BYTES, TN, 0, r, 2 * h + jl);
+        al[h][nb] = frag(sbase + A_BYTES, TN, 1, r, 2 * h + jl);
       }
 #pragma unroll
-      for (int mb = 0; mb < 2; ++mb) {
-        bh[mb] = *(const bf16x8_t*)&sA[0][wm + mb * 32 + (lane & 31)][kk];
-        bl[mb] = *(const bf16x8_t*)&sA[1][wm + mb * 32 + (lane & 31)][kk];
+      for (int mb = 0; mb < MB; ++mb) {
+        const int r = wm + mb * 32 + (lane & 31);
+        bh[h][mb] = frag(sbase, TM, 0, r, 2 * h + jl);
+        bl[h][mb] = frag(sbase, TM, 1, r, 2 * h + jl);
       }
-#pragma unroll
-      for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb) {
-          acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[nb], bh[mb], acc[nb][mb], 0, 0, 0);
-          acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[nb], bl[mb], acc[nb][mb], 0, 0, 0);
-          acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[nb], bh[mb], acc[nb][mb], 0, 0, 0);
-        }
     }
-    __syncthreads();
+    __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
+    raw_barrier();  // every wave has its fragments of step kt: the buffer may be refilled
+    if (STAGES == 1 && kt + 1 < kt1) issue(kt + 1);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+          acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[h][nb], bh[h][mb], acc[nb][mb], 0, 0, 0);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+          acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[h][nb], bl[h][mb], acc[nb][mb], 0, 0, 0);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+          acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[h][nb], bh[h][mb], acc[nb][mb], 0, 0, 0);
+    }
   }
   // epilogue: lane holds m = mbase + (lane & 31), n = nbase + 8*(r/4) + 4*(lane/32) + r%4
-  const int64_t m0 = tm * PK_ROWS + wm, n0 = tn * PK_ROWS + wn;
+  const int64_t m0 = tm * TM + wm, n0 = tn * TN + wn;
   if (d.work) {
     float* W = d.work + ((ks * d.B + batch) * d.M) * d.N;
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb)
+    for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-      for (int mb = 0; mb < 2; ++mb)
+      for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int64_t m = m0 + mb * 32 + (lane & 31);
@@ -241,9 +303,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_packed_bf16x3(PackedGemmDesc d)
     const float sx = (float)d.sx.re, sy = (float)d.sy.re;
     float* Y = (float*)d.y + batch * d.syb;
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb)
+    for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-      for (int mb = 0; mb < 2; ++mb)
+      for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int64_t m = m0 + mb * 32 + (lane & 31);
@@ -260,9 +322,173 @@ __global__ __launch_bounds__(256, 2) void k_gemm_packed_bf16x3(PackedGemmDesc d)
   // slots.  Any other epilogue goes through the work buffer and k_packed_reduce.
   float* Y = (float*)d.y + 2 * batch * d.syb;
 #pragma unroll
-  for (int nb = 0; nb < 2; ++nb)
+  for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t m = m0 + mb * 32 + (lane & 31);
+        const int64_t n = n0 + nb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (m < d.M && n < d.N) {
+          const bool im = n >= d.N0;
+          const int64_t nn = im ? n - d.N0 : n;
+          Y[2 * (m * d.sym + nn * d.syn) + (im ? 1 : 0)] = acc[nb][mb][r];
+        }
+      }
+}
+
+// Software-pipelined variant (two LDS stages, fragments double-buffered in registers).
+// Per K-step kt (two 16-k sub-steps s0, s1), with the fragments of s0(kt) already loaded:
+//   read s1(kt) | MFMAs of s0(kt)
+//   wait: copy of kt+1 landed, own reads done; barrier
+//   issue the copy of kt+2 into the stage kt just vacated
+//   read s0(kt+1) | MFMAs of s1(kt)
+// so LDS reads are always in flight under MFMAs and there is one barrier per K-step.
+template <int TM, int TN>
+__global__ __launch_bounds__(256) void k_gemm_packed_bf16x3_pipe(PackedGemmDesc d) {
+  constexpr int MB = TM / 64, NB = TN / 64;
+  constexpr int A_BYTES = TM * PK_K * 2 * 2, B_BYTES = TN * PK_K * 2 * 2;
+  constexpr int STAGE = A_BYTES + B_BYTES;
+  constexpr int CHUNKS = STAGE / 1024;
+  constexpr int CPW = CHUNKS / 4;
+  static_assert(CHUNKS % 4 == 0, "copies must split evenly over 4 waves");
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * STAGE];
+  int64_t tm, tn, batch, ks;
+  pk_map(blockIdx.x, gridDim.x, d, tm, tn, batch, ks);
+  const int t = threadIdx.x;
+  const int lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
+  const int64_t kt0 = ks * d.kt_per_split;
+  int64_t kt1 = kt0 + d.kt_per_split;
+  if (kt1 > d.tiles_k) kt1 = d.tiles_k;
+  const uint8_t* ga = (const uint8_t*)(d.a + ((batch * d.tiles_m + tm) * d.tiles_k) * (int64_t)pk_tile_elems(TM));
+  const uint8_t* gb = (const uint8_t*)(d.b + ((batch * d.tiles_n + tn) * d.tiles_k) * (int64_t)pk_tile_elems(TN));
+  auto issue = [&](int64_t kt) __attribute__((always_inline)) {
+    const int sbase = (int)(kt & 1) * STAGE;
+#pragma unroll
+    for (int i = 0; i < CPW; ++i) {
+      const int c = w * CPW + i;
+      const uint8_t* src = (c * 1024 < A_BYTES) ? ga + kt * A_BYTES + c * 1024
+                                                 : gb + kt * B_BYTES + (c * 1024 - A_BYTES);
+      const int loff = __builtin_amdgcn_readfirstlane(sbase + c * 1024);
+      __builtin_amdgcn_global_load_lds((const void*)(src + lane * 16),
+                                       (__attribute__((address_space(3))) void*)(lds + loff),
+                                       16, 0, 0);
+    }
+  };
+  const int jl = lane >> 5;
+  // fragment reads of sub-step h of the step held in stage sb
+  auto rd = [&](int sb, int h, bf16x8_t (&ah)[NB], bf16x8_t (&al)[NB], bf16x8_t (&bh)[MB],
+                bf16x8_t (&bl)[MB]) __attribute__((always_inline)) {
+    const int base = sb * STAGE;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int r = wn + nb * 32 + (lane & 31);
+      const int off = base + A_BYTES + r * 64 + 16 * ((2 * h + jl) ^ ((r >> 2) & 3));
+      ah[nb] = *(const bf16x8_t*)(lds + off);
+      al[nb] = *(const bf16x8_t*)(lds + off + TN * 64);
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      const int r = wm + mb * 32 + (lane & 31);
+      const int off = base + r * 64 + 16 * ((2 * h + jl) ^ ((r >> 2) & 3));
+      bh[mb] = *(const bf16x8_t*)(lds + off);
+      bl[mb] = *(const bf16x8_t*)(lds + off + TM * 64);
+    }
+  };
+  floatx16 acc[NB][MB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int j = 0; j < MB; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  auto mm = [&](const bf16x8_t (&ah)[NB], const bf16x8_t (&al)[NB], const bf16x8_t (&bh)[MB],
+                const bf16x8_t (&bl)[MB]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+        acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[nb], bh[mb], acc[nb][mb], 0, 0, 0);
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+        acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[nb], bl[mb], acc[nb][mb], 0, 0, 0);
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+        acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[nb], bh[mb], acc[nb][mb], 0, 0, 0);
+  };
+  bf16x8_t ah0[NB], al0[NB], bh0[MB], bl0[MB], ah1[NB], al1[NB], bh1[MB], bl1[MB];
+  if (kt0 < kt1) {
+    issue(kt0);
+    if (kt0 + 1 < kt1) issue(kt0 + 1);
+    if (kt0 + 1 < kt1) __builtin_amdgcn_s_waitcnt(waitcnt_vm(CPW));
+    else __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    raw_barrier();
+    rd((int)(kt0 & 1), 0, ah0, al0, bh0, bl0);  // a split-K slice may start on an odd step
+  }
+  // sched_barrier(0) fences the instruction scheduler so that each group of LDS reads stays
+  // ahead of the MFMAs it overlaps, and the MFMAs stay ahead of the next wait.
+  for (int64_t kt = kt0; kt < kt1; ++kt) {
+    const int sb = (int)(kt & 1);
+    // the fragments of sub-step 0 were read under the previous MFMAs; retire them here so
+    // that the reads below are not waited on before the MFMAs that do not need them
+    __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
+    __builtin_amdgcn_sched_barrier(0);
+    rd(sb, 1, ah1, al1, bh1, bl1);
+    __builtin_amdgcn_sched_barrier(0);
+    mm(ah0, al0, bh0, bl0);
+    __builtin_amdgcn_sched_barrier(0);
+    // copy of kt+1 landed (the copy of kt+2 is not issued yet), own reads of stage sb done
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0) & WAIT_LGKM0);
+    raw_barrier();
+    if (kt + 2 < kt1) issue(kt + 2);
+    if (kt + 1 < kt1) rd(sb ^ 1, 0, ah0, al0, bh0, bl0);
+    __builtin_amdgcn_sched_barrier(0);
+    mm(ah1, al1, bh1, bl1);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const int64_t m0 = tm * TM + wm, n0 = tn * TN + wn;
+  if (d.work) {
+    float* W = d.work + ((ks * d.B + batch) * d.M) * d.N;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t m = m0 + mb * 32 + (lane & 31);
+          const int64_t n = n0 + nb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          if (m < d.M && n < d.N) W[m + n * d.M] = acc[nb][mb][r];
+        }
+    return;
+  }
+  if (!d.cplx) {
+    const float sx = (float)d.sx.re, sy = (float)d.sy.re;
+    float* Y = (float*)d.y + batch * d.syb;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t m = m0 + mb * 32 + (lane & 31);
+          const int64_t n = n0 + nb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          if (m < d.M && n < d.N) {
+            float* yp = Y + m * d.sym + n * d.syn;
+            *yp = epilogue<OpsF32>(d.mode, sx, sy, acc[nb][mb][r], yp);
+          }
+        }
+    return;
+  }
+  float* Y = (float*)d.y + 2 * batch * d.syb;
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int64_t m = m0 + mb * 32 + (lane & 31);
@@ -309,14 +535,49 @@ bool packed_plain_overwrite(const GemmDesc& g) {
   return g.sx.re == 1.0 && g.sx.im == 0.0 && g.sy.re == 0.0 && g.sy.im == 0.0;
 }
 
-int64_t packed_bytes(int64_t rows, int64_t k, int64_t batch) {
-  const int64_t tr = (rows + PK_ROWS - 1) / PK_ROWS, tk = (k + PK_K - 1) / PK_K;
-  return batch * tr * tk * (int64_t)PK_TILE * 2;
+// Workgroup tile (TM x TN) for a packed GEMM.  OME_PACKED_TILE=128|256x128|128x256|256
+// forces one for measurements; by default large products use 256x128.
+static void packed_tiles(int64_t M, int64_t N, int& TM, int& TN) {
+  static int forced = -1;
+  if (forced < 0) {
+    const char* e = getenv("OME_PACKED_TILE");
+    forced = 0;
+    if (e) {
+      if (!strcmp(e, "128")) forced = 1;
+      else if (!strcmp(e, "256x128")) forced = 2;
+      else if (!strcmp(e, "128x256")) forced = 3;
+      else if (!strcmp(e, "256")) forced = 4;
+    }
+  }
+  switch (forced) {
+    case 1: TM = 128; TN = 128; return;
+    case 2: TM = 256; TN = 128; return;
+    case 3: TM = 128; TN = 256; return;
+    case 4: TM = 256; TN = 128; return;
+  }
+  TM = 128;
+  TN = 128;
+}
+
+int64_t packed_bytes_a(int64_t M, int64_t N, int64_t K, int64_t batch) {
+  int TM, TN;
+  packed_tiles(M, N, TM, TN);
+  return batch * ((M + TM - 1) / TM) * ((K + PK_K - 1) / PK_K) * (int64_t)pk_tile_elems(TM) * 2;
+}
+int64_t packed_bytes_b(int64_t M, int64_t N, int64_t K, int64_t batch) {
+  int TM, TN;
+  packed_tiles(M, N, TM, TN);
+  return batch * ((N + TN - 1) / TN) * ((K + PK_K - 1) / PK_K) * (int64_t)pk_tile_elems(TN) * 2;
+}
+int64_t packed_tiles_mn(int64_t M, int64_t N) {
+  int TM, TN;
+  packed_tiles(M, N, TM, TN);
+  return ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
 }
 
 static void launch_pack(const void* src, uint16_t* dst, int64_t R, int64_t K, int64_t B,
                         int64_t sr, int64_t sk, int64_t sb, int mode, int64_t R0, int64_t K0,
-                        hipStream_t st) {
+                        int TR, hipStream_t st) {
   PackDesc p{};
   p.src = src;
   p.dst = dst;
@@ -328,11 +589,12 @@ static void launch_pack(const void* src, uint16_t* dst, int64_t R, int64_t K, in
   p.R0 = R0;
   p.K0 = K0;
   p.mode = mode;
-  p.tiles_r = (R + PK_ROWS - 1) / PK_ROWS;
+  p.tiles_r = (R + TR - 1) / TR;
   p.tiles_k = (K + PK_K - 1) / PK_K;
   p.rfast = (sr == 1 && sk != 1) ? 1 : 0;
   const int64_t blocks = p.tiles_r * p.tiles_k * B;
-  hipLaunchKernelGGL(k_pack_bf16x3, dim3((uint32_t)blocks), dim3(256), 0, st, p);
+  if (TR == 256) hipLaunchKernelGGL(k_pack_bf16x3<256>, dim3((uint32_t)blocks), dim3(256), 0, st, p);
+  else hipLaunchKernelGGL(k_pack_bf16x3<128>, dim3((uint32_t)blocks), dim3(256), 0, st, p);
 }
 
 // Real F32 or complex C32 GEMM on packed BF16x3 operands.  pa/pb: workspace of
@@ -343,9 +605,11 @@ void launch_gemm_packed(int dt, const GemmDesc& g, void* pa, void* pb, float* wo
   const bool cplx = (dt == C32);
   PackedGemmDesc d{};
   const int64_t M = g.M, N = cplx ? 2 * g.N : g.N, K = cplx ? 2 * g.K : g.K;
+  int TM, TN;
+  packed_tiles(M, N, TM, TN);
   // A' rows m (stride sam), k (stride sak); B' rows n (stride sbn), k (stride sbk)
-  launch_pack(g.a, (uint16_t*)pa, M, K, g.B, g.sam, g.sak, g.sab, cplx ? 1 : 0, g.M, g.K, st);
-  launch_pack(g.b, (uint16_t*)pb, N, K, g.B, g.sbn, g.sbk, g.sbb, cplx ? 2 : 0, g.N, g.K, st);
+  launch_pack(g.a, (uint16_t*)pa, M, K, g.B, g.sam, g.sak, g.sab, cplx ? 1 : 0, g.M, g.K, TM, st);
+  launch_pack(g.b, (uint16_t*)pb, N, K, g.B, g.sbn, g.sbk, g.sbb, cplx ? 2 : 0, g.N, g.K, TN, st);
   d.a = (const uint16_t*)pa;
   d.b = (const uint16_t*)pb;
   d.y = g.y;
@@ -353,8 +617,8 @@ void launch_gemm_packed(int dt, const GemmDesc& g, void* pa, void* pb, float* wo
   d.N = N;
   d.K = K;
   d.B = g.B;
-  d.tiles_m = (M + PK_ROWS - 1) / PK_ROWS;
-  d.tiles_n = (N + PK_ROWS - 1) / PK_ROWS;
+  d.tiles_m = (M + TM - 1) / TM;
+  d.tiles_n = (N + TN - 1) / TN;
   d.tiles_k = (K + PK_K - 1) / PK_K;
   d.sym = g.sym;
   d.syn = g.syn;
@@ -374,7 +638,26 @@ void launch_gemm_packed(int dt, const GemmDesc& g, void* pa, void* pb, float* wo
   const bool via_work = ksplit > 1 || (cplx && !packed_plain_overwrite(g));
   d.work = via_work ? work : nullptr;
   const int64_t blocks = d.tiles_m * d.tiles_n * d.B * ksplit;
-  hipLaunchKernelGGL(k_gemm_packed_bf16x3, dim3((uint32_t)blocks), dim3(256), 0, st, d);
+  static int stages = -1;
+  if (stages < 0) {
+    const char* e = getenv("OME_PACKED_STAGES");
+    // 1: single LDS buffer (default); 2, 3: ring of LDS buffers; 4: software-pipelined
+    stages = (e && atoi(e) >= 1 && atoi(e) <= 4) ? atoi(e) : 1;
+  }
+  const dim3 grid((uint32_t)blocks), blk(256);
+  if (TM == 256 && TN == 128) {
+    if (stages != 2) hipLaunchKernelGGL((k_gemm_packed_bf16x3<256, 128, 1>), grid, blk, 0, st, d);
+    else hipLaunchKernelGGL((k_gemm_packed_bf16x3<256, 128, 2>), grid, blk, 0, st, d);
+  } else if (TN == 256 && TM == 128) {
+    if (stages != 2) hipLaunchKernelGGL((k_gemm_packed_bf16x3<128, 256, 1>), grid, blk, 0, st, d);
+    else hipLaunchKernelGGL((k_gemm_packed_bf16x3<128, 256, 2>), grid, blk, 0, st, d);
+  } else if (stages == 4) {
+    hipLaunchKernelGGL((k_gemm_packed_bf16x3_pipe<128, 128>), grid, blk, 0, st, d);
+  } else {
+    if (stages == 1) hipLaunchKernelGGL((k_gemm_packed_bf16x3<128, 128, 1>), grid, blk, 0, st, d);
+    else if (stages == 3) hipLaunchKernelGGL((k_gemm_packed_bf16x3<128, 128, 3>), grid, blk, 0, st, d);
+    else hipLaunchKernelGGL((k_gemm_packed_bf16x3<128, 128, 2>), grid, blk, 0, st, d);
+  }
   if (via_work) {
     const int64_t outs = cplx ? g.M * g.N * g.B : M * N * g.B;
     int64_t gr = (outs + 255) / 256;

@@ -39,7 +39,9 @@ int64_t mfma_workspace(int dt, const GemmDesc& d, int ksplit);
 void launch_gemm_mfma(int prec, GemmDesc d, int ksplit, hipStream_t st);
 
 // Packed BF16x3 GEMM (Float32, or ComplexF32 realified into one real GEMM).
-int64_t packed_bytes(int64_t rows, int64_t k, int64_t batch);
+int64_t packed_bytes_a(int64_t M, int64_t N, int64_t K, int64_t batch);
+int64_t packed_bytes_b(int64_t M, int64_t N, int64_t K, int64_t batch);
+int64_t packed_tiles_mn(int64_t M, int64_t N);
 bool packed_plain_overwrite(const GemmDesc& g);
 void launch_gemm_packed(int dt, const GemmDesc& g, void* pa, void* pb, float* work, int ksplit,
                         hipStream_t st);

@@ -227,12 +227,12 @@ static Status do_unary(ome_ctx* ctx, int dt, const void* x, const std::vector<in
 static Status run_gemm_packed(ome_ctx* ctx, int dt, const GemmDesc& d, hipStream_t st) {
   const bool cplx = dt == C32;
   const int64_t M = d.M, N = cplx ? 2 * d.N : d.N, K = cplx ? 2 * d.K : d.K;
-  const int64_t tiles = ((M + 127) / 128) * ((N + 127) / 128) * d.B;
+  const int64_t tiles = packed_tiles_mn(M, N) * d.B;
   const int64_t ktiles = (K + 31) / 32;
   int ks = 1;
-  while (tiles * ks < 512 && ktiles / (ks * 2) >= 8) ks *= 2;
-  Temp pa(ctx, (size_t)packed_bytes(M, K, d.B));
-  Temp pb(ctx, (size_t)packed_bytes(N, K, d.B));
+  while (tiles * ks < 256 && ktiles / (ks * 2) >= 8) ks *= 2;
+  Temp pa(ctx, (size_t)packed_bytes_a(M, N, K, d.B));
+  Temp pb(ctx, (size_t)packed_bytes_b(M, N, K, d.B));
   const bool via_work = ks > 1 || (cplx && !packed_plain_overwrite(d));
   Temp work(ctx, via_work ? (size_t)ks * d.B * M * N * 4 : 0);
   if (!pa.p || !pb.p || (via_work && !work.p))
