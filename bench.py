@@ -3,18 +3,27 @@
 Headline workload: ein"abcd,cdef->abef" with every label of size 64 (two 16.7M-element
 Float32 tensors; one 4096^3 GEMM-shaped contraction), evaluated by the MI355X backend with
 the 3-term BF16 split on the matrix cores.  A step = one einsum forward over inputs already
-resident in HBM.  value = algorithmic TFLOP/s (2*64^6 flop per step).
+resident in HBM (split/pack of both operands + the packed GEMM).  value = algorithmic
+TFLOP/s (2*64^6 flop per step).
+
+roofline: the dominant kernel is the packed BF16x3 GEMM.  Its launches are bracketed by HIP
+events on the stream they run on (the library's kernel-timing hook) during the timed region;
+achieved = 2*64^6 flop per launch / average launch duration, against the FP32-equivalent
+peak of the 3-product split (dense BF16 MFMA peak / 3).  traffic = HBM bytes per launch from
+the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this kernel, when present.
 
 Also reported in the same line: the same contraction in plain FP32 MFMA and its error vs
-the FP64 oracle; ijk,jkl->il at 64^3; the 100-vertex 3-regular ComplexF32 network contracted
-to a scalar by the MI355X backend (graph replay) next to the plain per-step GPU path; the
-CPU baseline (the oracle's numpy restatement of OMEinsum's DefaultBackend on this host).
+FP64; ijk,jkl->il at 64^3; the 100-vertex 3-regular ComplexF32 network contracted to a scalar
+by the MI355X backend (graph replay) next to the plain per-step GPU path; the CPU baseline
+(the oracle's numpy restatement of OMEinsum's DefaultBackend on this host).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|c4]
+                       [--no-cpu] [--copy-calibration]
 """
 import argparse
 import json
 import os
+import re
 import statistics
 import sys
 import time
@@ -28,19 +37,23 @@ PEAK_BF16 = 2516.6      # TFLOP/s dense (MI355X datasheet)
 PEAK_FP32 = 157.3
 PEAK_FP64 = 78.6
 PEAK_HBM = 8000.0       # GB/s
+C4_FLOPS = 2.0 * 64 ** 6
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "gemm_packed_bf16x3_abcd_cdef_64_traffic.md")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_events(fn, steps, warmup):
+def time_events(fn, steps, warmup, after_warmup=None):
     """Average milliseconds per call: HIP events on torch's current stream (the library
     launches on that stream) bracketing `steps` calls after `warmup` calls."""
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
+    if after_warmup is not None:
+        after_warmup()
     s = torch.cuda.Event(enable_timing=True)
     e = torch.cuda.Event(enable_timing=True)
     s.record()
@@ -51,11 +64,25 @@ def time_events(fn, steps, warmup):
     return s.elapsed_time(e) / steps
 
 
-def bench_c4(steps, warmup):
+def read_traffic():
+    """HBM bytes per launch of the packed GEMM on the headline workload, from the committed
+    rocprofv3 counter summary (line `traffic_bytes_per_launch: N`), or None."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                m = re.match(r"\s*traffic_bytes_per_launch:\s*([0-9.eE+]+)", line)
+                if m:
+                    return float(m.group(1))
+    except OSError:
+        pass
+    return None
+
+
+def bench_c4(steps, warmup, precs=("bf16x3", "fp32")):
     import numpy as np
     import torch
     import omeinsum_mi355x as om
-    from oracle import default_backend as db
+    from omeinsum_mi355x import _lib
 
     g = torch.Generator(device="cuda").manual_seed(4000)
     a = torch.randn(64, 64, 64, 64, device="cuda", generator=g)
@@ -63,13 +90,26 @@ def bench_c4(steps, warmup):
     code = om.ein("abcd,cdef->abef")
     sd = om.get_size_dict(code.ixs, (a, b))
     y = torch.empty(64, 64, 64, 64, device="cuda")
-    flops = 2.0 * 64 ** 6
+    ctx = _lib.context()
     out = {}
-    for prec in ("bf16x3", "fp32"):
+    for prec in precs:
         om.set_einsum_backend(om.MI355XBackend(precision=prec))
         fn = lambda: om.einsum_inplace(code, (a, b), y, True, False, sd)
-        ms = time_events(fn, steps, warmup)
-        # error of the result against the FP64 oracle on a sample of output rows
+        kc = "gemm_packed" if prec == "bf16x3" else "gemm_mfma"
+
+        def clear():
+            for k in ("gemm_packed", "gemm_mfma", "pack"):
+                ctx.kernel_timing(k)
+
+        ctx.set_option(_lib.OPT_KERNEL_TIMING, 1)
+        try:
+            ms = time_events(fn, steps, warmup, after_warmup=clear)
+            k_ms, k_n = ctx.kernel_timing(kc)
+            p_ms, p_n = ctx.kernel_timing("pack")
+        finally:
+            ctx.set_option(_lib.OPT_KERNEL_TIMING, 0)
+        kern_ms = k_ms / max(k_n, 1)
+        # error of the result against FP64 on a sample of output rows
         fn()
         torch.cuda.synchronize()
         an = a.cpu().numpy().astype(np.float64).reshape(4096, 4096)
@@ -78,8 +118,13 @@ def bench_c4(steps, warmup):
         ref = an[rows] @ bn           # row-major: abcd -> (ab),(cd)
         got = y.cpu().numpy().reshape(4096, 4096)[rows].astype(np.float64)
         err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-        out[prec] = {"ms": ms, "tflops": flops / ms / 1e9, "rel_err_fp64": err}
-        log("C4 %s: %.3f ms, %.1f TFLOP/s, rel err vs FP64 %.2e" % (prec, ms, flops / ms / 1e9, err))
+        out[prec] = {"ms": ms, "tflops": C4_FLOPS / ms / 1e9, "rel_err_fp64": err,
+                     "gemm_kernel_us": kern_ms * 1e3, "gemm_kernel_launches": k_n,
+                     "gemm_kernel_tflops": C4_FLOPS / kern_ms / 1e9 if k_n else None,
+                     "pack_kernel_us": (p_ms / p_n * 1e3) if p_n else None}
+        log("C4 %s: %.3f ms/step, %.1f TFLOP/s; GEMM kernel %.1f us (%d launches, %.1f TFLOP/s); "
+            "rel err vs FP64 %.2e" % (prec, ms, C4_FLOPS / ms / 1e9, kern_ms * 1e3, k_n,
+                                      C4_FLOPS / kern_ms / 1e9 if k_n else 0, err))
     om.set_einsum_backend(om.DefaultBackend())
     return out, (a, b)
 
@@ -119,6 +164,7 @@ def bench_c2(steps, warmup):
     xs = networks.gaussian_tensors(code, sd, seed=2001, dtype=np.complex64)
     tree = co.optimize_code(code, sd)
     dxs = [torch.from_numpy(x).cuda() for x in xs]
+    # FP64 reference of the same tree (the oracle's per-step restatement, in complex128)
     ref = complex(db.nested_einsum(tree.to_plain(), [x.astype(np.complex128) for x in xs], sd)[()])
     out = {"n_vertices": n, "flops_log2": float(np.log2(8 * co.flop(tree, sd)))}
     y = torch.empty((), dtype=torch.complex64, device="cuda")
@@ -143,11 +189,35 @@ def bench_c2(steps, warmup):
     return out
 
 
+def copy_calibration(reps=5):
+    """A streaming copy of 1 GiB through one elementwise kernel (y = x + 0: each element read
+    once, written once), repeated: the reading of the FETCH_SIZE / WRITE_SIZE counters for a
+    kernel that moves exactly 1 GiB each way."""
+    import torch
+    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        torch.add(x, 0.0, out=y)
+    torch.cuda.synchronize()
+    log("copy calibration: %d kernels, each reading and writing %d bytes" % (reps, x.numel() * 4))
+
+
+def blas_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        n = [i["num_threads"] for i in threadpool_info() if i.get("user_api") == "blas"]
+        if n:
+            return int(max(n))
+    except Exception:  # noqa: BLE001 - fall back to the visible core count
+        pass
+    return os.cpu_count()
+
+
 def cpu_baseline(ab):
     """The oracle's restatement of OMEinsum's DefaultBackend (analyze_binary + permutes +
     OpenBLAS GEMM through numpy) on this host, on the same 64^4 Float32 inputs.  Bounded
-    sample: 5 timed repetitions after one warm-up."""
-    import numpy as np
+    sample: up to 5 timed repetitions after one warm-up, stopping after ~25 s."""
     from oracle import default_backend as db
     a = ab[0].cpu().numpy()
     b = ab[1].cpu().numpy()
@@ -162,10 +232,11 @@ def cpu_baseline(ab):
         if time.time() > t_end:
             break
     med = statistics.median(ts)
-    return {"value": 2.0 * 64 ** 6 / med / 1e12, "unit": "TFLOP/s", "cores": os.cpu_count(),
+    return {"value": round(C4_FLOPS / med / 1e12, 4), "unit": "TFLOP/s", "cores": blas_threads(),
             "kind": "port",
-            "sample": "abcd,cdef->abef 64^4 Float32, oracle DefaultBackend restatement "
-                      "(numpy/OpenBLAS sgemm), median of %d runs = %.3f s" % (len(ts), med)}
+            "sample": "the full abcd,cdef->abef 64^4 Float32 contraction, oracle DefaultBackend "
+                      "restatement (numpy permutes + OpenBLAS sgemm), median of %d runs = %.3f s"
+                      % (len(ts), med)}
 
 
 def main():
@@ -173,10 +244,12 @@ def main():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--workload", choices=("all", "c4"), default="all")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-calibration", action="store_true")
     args = p.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X (no multi-GPU "
+        print("bench.py measures one GPU: this backend runs on a single MI355X (no multi-GPU "
               "path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
@@ -185,11 +258,18 @@ def main():
         sys.exit(1)
     import omeinsum_mi355x as om  # noqa: F401
 
-    c4, ab = bench_c4(args.steps, args.warmup)
-    c1 = bench_c1(args.steps, args.warmup)
-    c2 = bench_c2(max(5, args.steps // 2), 2)
+    precs = ("bf16x3",) if args.workload == "c4" else ("bf16x3", "fp32")
+    c4, ab = bench_c4(args.steps, args.warmup, precs)
+    extra = {"c4": c4}
+    if args.workload == "all":
+        extra["c1_ijk_jkl_il_64"] = bench_c1(args.steps, args.warmup)
+        extra["c2_network_100"] = bench_c2(max(5, args.steps // 2), 2)
+    if args.copy_calibration:
+        copy_calibration()
     cpu = None if args.no_cpu else cpu_baseline(ab)
     head = c4["bf16x3"]
+    peak = PEAK_BF16 / 3
+    kt = head["gemm_kernel_tflops"]
     line = {
         "metric": "einsum TFLOP/s (% MFMA peak) vs plain GPU path; p50/p99 TN-request latency",
         "value": round(head["tflops"], 2),
@@ -205,14 +285,15 @@ def main():
         "data": "synthetic (seeded normal inputs)",
         "config": {"workload": "ein\"abcd,cdef->abef\", all labels 64 (two 64^4 Float32 tensors), "
                                "MI355XBackend(precision=bf16x3)", "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": round(head["tflops"], 2),
-                     "peak": round(PEAK_BF16 / 3, 1), "unit": "TFLOP/s",
-                     "frac": round(head["tflops"] / (PEAK_BF16 / 3), 4), "traffic": None,
+        "roofline": {"bound": "mfma", "achieved": round(kt, 2) if kt else None,
+                     "peak": round(peak, 1), "unit": "TFLOP/s",
+                     "frac": round(kt / peak, 4) if kt else None, "traffic": read_traffic(),
+                     "kernel": "k_gemm_packed_bf16x3 (avg %.1f us over %d launches)"
+                               % (head["gemm_kernel_us"], head["gemm_kernel_launches"]),
                      "peak_note": "FP32-equivalent peak of the 3-product BF16 split = dense BF16 "
-                                  "MFMA peak 2516.6 / 3; frac vs the FP32 MFMA peak 157.3 = %.3f"
-                                  % (head["tflops"] / PEAK_FP32)},
+                                  "MFMA peak 2516.6 / 3"},
         "cpu_baseline": cpu,
-        "extra": {"c4": c4, "c1_ijk_jkl_il_64": c1, "c2_network_100": c2},
+        "extra": extra,
     }
     print(json.dumps(line), flush=True)
 

@@ -80,8 +80,21 @@ typedef enum ome_option {
   OME_OPT_DIRECT_GEMM = 3,  /* 1 (default): strided GEMM reads operands in place when layout allows */
   OME_OPT_USE_GRAPHS = 4,   /* 1 (default): tree plans replay a captured hipGraph */
   OME_OPT_AUTOTUNE = 5,     /* 1 (default): time GEMM tile candidates on first use of a shape class */
-  OME_OPT_GEMM_KERNEL = 6   /* force a GEMM kernel family (-1 = automatic; for tests) */
+  OME_OPT_GEMM_KERNEL = 6,  /* force a GEMM kernel family (-1 = automatic; for tests) */
+  OME_OPT_KERNEL_TIMING = 7 /* 1: bracket kernel launches with HIP events (process-wide) */
 } ome_option;
+
+/* Kernel classes of ome_get_kernel_timing. */
+typedef enum ome_kernel_class {
+  OME_KC_GEMM_PACKED = 0, /* BF16x3 GEMM on packed hi/lo operand tiles */
+  OME_KC_PACK = 1,        /* FP32 -> BF16 hi/lo split and tile packing */
+  OME_KC_GEMM_MFMA = 2,   /* FP32 / FP64 / BF16x3 MFMA GEMM on strided operands */
+  OME_KC_GEMM_VALU = 3,   /* vector-ALU GEMM (Int64, tropical, complex, small shapes) */
+  OME_KC_PERMUTE = 4,     /* permutedims / tensorpermute */
+  OME_KC_WALK = 5,        /* index walk (loop_einsum, unary reductions) */
+  OME_KC_GROUPED = 6,     /* grouped GEMM */
+  OME_KC_OTHER = 7        /* split-K reductions and the rest */
+} ome_kernel_class;
 
 /* A scalar for sx / sy / alpha / beta.  Float types read re (and im for complex);
  * OME_I64 reads i; tropical types read re as the tropical content (so the tropical one
@@ -141,6 +154,11 @@ int ome_set_option(ome_ctx* ctx, int key, int64_t value);
 int ome_get_option(ome_ctx* ctx, int key, int64_t* value);
 int ome_get_device_info(ome_ctx* ctx, ome_device_info* info);
 int ome_synchronize(ome_ctx* ctx);
+/* No reference counterpart: measurement hook.  While OME_OPT_KERNEL_TIMING is 1, every launch
+ * of a kernel class is bracketed by HIP events on the stream it runs on (launches captured into
+ * a graph are not timed).  Waits for the recorded launches of kclass, returns their summed
+ * duration and their number, and clears them. */
+int ome_get_kernel_timing(ome_ctx* ctx, int kclass, double* total_ms, int64_t* launches);
 
 /* ---- memory pool (replaces get_output_array / similar / unsafe_free!:
  *      src/loop_einsum.jl:48-61, ext/AMDGPUExt.jl:27-40,124-126) ------------------------ */

@@ -90,6 +90,92 @@ ome_pool_stats Pool::stats() {
   return s;
 }
 
+// ---- kernel timing ---------------------------------------------------------------------------
+
+namespace {
+struct TimingState {
+  std::mutex mu;
+  bool on = false;
+  std::vector<hipEvent_t> spare;
+  struct Rec {
+    int kc;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  hipEvent_t get() {
+    if (!spare.empty()) {
+      hipEvent_t e = spare.back();
+      spare.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+};
+TimingState& timing() {
+  static TimingState t;
+  return t;
+}
+}  // namespace
+
+void timing_enable(bool on) {
+  std::lock_guard<std::mutex> g(timing().mu);
+  timing().on = on;
+}
+bool timing_enabled() { return timing().on; }
+
+TimedScope::TimedScope(int k, hipStream_t s) : kc(k), st(s) {
+  TimingState& t = timing();
+  if (!t.on) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+  std::lock_guard<std::mutex> g(t.mu);
+  start = t.get();
+  if (start && hipEventRecord(start, st) != hipSuccess) {
+    t.spare.push_back(start);
+    start = nullptr;
+  }
+}
+
+TimedScope::~TimedScope() {
+  if (!start) return;
+  TimingState& t = timing();
+  std::lock_guard<std::mutex> g(t.mu);
+  hipEvent_t stop = t.get();
+  if (stop && hipEventRecord(stop, st) == hipSuccess) {
+    t.recs.push_back({kc, start, stop});
+  } else {
+    t.spare.push_back(start);
+    if (stop) t.spare.push_back(stop);
+  }
+}
+
+void timing_collect(int kc, double* total_ms, int64_t* launches) {
+  TimingState& t = timing();
+  std::lock_guard<std::mutex> g(t.mu);
+  double ms = 0;
+  int64_t n = 0;
+  std::vector<TimingState::Rec> keep;
+  for (auto& r : t.recs) {
+    if (r.kc != kc) {
+      keep.push_back(r);
+      continue;
+    }
+    float e = 0.f;
+    hipEventSynchronize(r.b);
+    if (hipEventElapsedTime(&e, r.a, r.b) == hipSuccess) {
+      ms += e;
+      ++n;
+    }
+    t.spare.push_back(r.a);
+    t.spare.push_back(r.b);
+  }
+  t.recs.swap(keep);
+  *total_ms = ms;
+  *launches = n;
+}
+
 }  // namespace ome
 
 // ---- error plumbing -------------------------------------------------------------------------
@@ -185,6 +271,7 @@ int ome_set_option(ome_ctx* ctx, int key, int64_t v) {
     case OME_OPT_GEMM_KERNEL:
       if (v < -1 || v > 2) return fail(ctx, OME_ERR_INVALID_ARG, "gemm kernel must be -1..2");
       ctx->opt.gemm_kernel = (int)v; break;
+    case OME_OPT_KERNEL_TIMING: timing_enable(v != 0); break;
     default: return fail(ctx, OME_ERR_INVALID_ARG, "unknown option");
   }
   return OME_OK;
@@ -201,8 +288,18 @@ int ome_get_option(ome_ctx* ctx, int key, int64_t* v) {
     case OME_OPT_USE_GRAPHS: *v = ctx->opt.use_graphs; break;
     case OME_OPT_AUTOTUNE: *v = ctx->opt.autotune; break;
     case OME_OPT_GEMM_KERNEL: *v = ctx->opt.gemm_kernel; break;
+    case OME_OPT_KERNEL_TIMING: *v = timing_enabled() ? 1 : 0; break;
     default: return fail(ctx, OME_ERR_INVALID_ARG, "unknown option");
   }
+  return OME_OK;
+}
+
+int ome_get_kernel_timing(ome_ctx* ctx, int kclass, double* total_ms, int64_t* launches) {
+  OME_CHECK_CTX(ctx);
+  if (!total_ms || !launches) return fail(ctx, OME_ERR_INVALID_ARG, "null output pointer");
+  if (kclass < 0 || kclass >= KC_COUNT) return fail(ctx, OME_ERR_INVALID_ARG, "unknown kernel class");
+  hipSetDevice(ctx->device);
+  timing_collect(kclass, total_ms, launches);
   return OME_OK;
 }
 

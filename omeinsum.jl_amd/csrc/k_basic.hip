@@ -312,6 +312,7 @@ __global__ __launch_bounds__(256) void k_permute_tiled(PermDesc d, int t) {
 void launch_permute(int dt, const PermDesc& d, hipStream_t st) {
   if (d.n <= 0) return;
   OME_DRY_RETURN(1);
+  TimedScope ts(KC_PERMUTE, st);
   // tiled path when some output dim t > 0 is the input's contiguous dim and both extents
   // are large enough to fill a tile
   int t = -1;
@@ -443,9 +444,14 @@ void launch_walk(int dt, const WalkDesc& d, hipStream_t st) {
   OME_DRY_RETURN((d.nsplit > 1 ? 2 : 1));
   const int64_t total = d.ny * d.nsplit;
   OME_DISPATCH(dt, O, {
-    hipLaunchKernelGGL(k_walk<O>, dim3(grid_for(total, 256, 1 << 20)), dim3(256), 0, st, d);
-    if (d.nsplit > 1)
+    {
+      TimedScope ts(KC_WALK, st);
+      hipLaunchKernelGGL(k_walk<O>, dim3(grid_for(total, 256, 1 << 20)), dim3(256), 0, st, d);
+    }
+    if (d.nsplit > 1) {
+      TimedScope ts(KC_OTHER, st);
       hipLaunchKernelGGL(k_walk_reduce<O>, dim3(grid_for(d.ny)), dim3(256), 0, st, d);
+    }
   });
 }
 

@@ -147,6 +147,7 @@ void launch_gemm_valu(int dt, GemmDesc d, int kernel, int ksplit, hipStream_t st
     if (kernel == GEMM_VALU_DIRECT) {
       d.ksplit = 1;
       int order_m_fast = (d.sym == 1 || d.N == 1) ? 1 : 0;
+      TimedScope ts(KC_GEMM_VALU, st);
       hipLaunchKernelGGL(k_gemm_direct<O>, dim3(grid_cap(outputs)), dim3(256), 0, st, d,
                          order_m_fast);
     } else {
@@ -159,8 +160,12 @@ void launch_gemm_valu(int dt, GemmDesc d, int kernel, int ksplit, hipStream_t st
       d.ksplit = ksplit;
       const int64_t tiles_m = (d.M + 63) / 64, tiles_n = (d.N + 63) / 64;
       const int64_t blocks = tiles_m * tiles_n * d.B * ksplit;
-      hipLaunchKernelGGL(k_gemm_tile<O>, dim3((uint32_t)blocks), dim3(256), 0, st, d, tiles_m,
-                         tiles_n, kchunk);
+      {
+        TimedScope ts(KC_GEMM_VALU, st);
+        hipLaunchKernelGGL(k_gemm_tile<O>, dim3((uint32_t)blocks), dim3(256), 0, st, d, tiles_m,
+                           tiles_n, kchunk);
+      }
+      TimedScope ts(KC_OTHER, st);
       if (ksplit > 1)
         hipLaunchKernelGGL(k_gemm_splitk_reduce<O>, dim3(grid_cap(outputs)), dim3(256), 0, st, d);
     }

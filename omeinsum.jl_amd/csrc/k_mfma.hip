@@ -509,6 +509,8 @@ void launch_gemm_mfma(int prec, GemmDesc d, int ksplit, hipStream_t st) {
   else if (ak) hipLaunchKernelGGL((KER<true, false>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk, va, vb);  \
   else if (bk) hipLaunchKernelGGL((KER<false, true>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk, va, vb);  \
   else hipLaunchKernelGGL((KER<false, false>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk, va, vb);
+  {
+  TimedScope ts(KC_GEMM_MFMA, st);
   if (prec == PREC_BF16X3) {
     OME_MFMA_LAUNCH(k_mfma_bf16x3)
   } else if (prec == PREC_FP32) {
@@ -516,8 +518,10 @@ void launch_gemm_mfma(int prec, GemmDesc d, int ksplit, hipStream_t st) {
   } else {
     OME_MFMA_LAUNCH(k_mfma_f64)
   }
+  }
 #undef OME_MFMA_LAUNCH
   if (ksplit > 1) {
+    TimedScope ts(KC_OTHER, st);
     const int64_t outputs = d.M * d.N * d.B;
     int g = (int)((outputs + 255) / 256);
     if (g > 65536) g = 65536;

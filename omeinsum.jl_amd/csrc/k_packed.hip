@@ -22,7 +22,6 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int PK_K = 32;          // k per packed tile
-constexpr int PK_LDK = PK_K + 8;  // LDS row stride (bf16): 80 bytes, conflict-free b128 reads
 __host__ __device__ constexpr int pk_tile_elems(int rows) { return 2 * rows * PK_K; }
 
 __device__ __forceinline__ uint16_t f2bf_rn(float x) {
@@ -593,6 +592,7 @@ static void launch_pack(const void* src, uint16_t* dst, int64_t R, int64_t K, in
   p.tiles_k = (K + PK_K - 1) / PK_K;
   p.rfast = (sr == 1 && sk != 1) ? 1 : 0;
   const int64_t blocks = p.tiles_r * p.tiles_k * B;
+  TimedScope ts(KC_PACK, st);
   if (TR == 256) hipLaunchKernelGGL(k_pack_bf16x3<256>, dim3((uint32_t)blocks), dim3(256), 0, st, p);
   else hipLaunchKernelGGL(k_pack_bf16x3<128>, dim3((uint32_t)blocks), dim3(256), 0, st, p);
 }
@@ -645,6 +645,8 @@ void launch_gemm_packed(int dt, const GemmDesc& g, void* pa, void* pb, float* wo
     stages = (e && atoi(e) >= 1 && atoi(e) <= 4) ? atoi(e) : 1;
   }
   const dim3 grid((uint32_t)blocks), blk(256);
+  {
+  TimedScope ts(KC_GEMM_PACKED, st);
   if (TM == 256 && TN == 128) {
     if (stages != 2) hipLaunchKernelGGL((k_gemm_packed_bf16x3<256, 128, 1>), grid, blk, 0, st, d);
     else hipLaunchKernelGGL((k_gemm_packed_bf16x3<256, 128, 2>), grid, blk, 0, st, d);
@@ -658,7 +660,9 @@ void launch_gemm_packed(int dt, const GemmDesc& g, void* pa, void* pb, float* wo
     else if (stages == 3) hipLaunchKernelGGL((k_gemm_packed_bf16x3<128, 128, 3>), grid, blk, 0, st, d);
     else hipLaunchKernelGGL((k_gemm_packed_bf16x3<128, 128, 2>), grid, blk, 0, st, d);
   }
+  }
   if (via_work) {
+    TimedScope ts(KC_OTHER, st);
     const int64_t outs = cplx ? g.M * g.N * g.B : M * N * g.B;
     int64_t gr = (outs + 255) / 256;
     if (gr > 65536) gr = 65536;

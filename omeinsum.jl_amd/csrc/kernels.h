@@ -17,6 +17,28 @@ LaunchMode& launch_mode();
     return;                          \
   }
 
+// Kernel timing (process-wide, off by default).  While on, a TimedScope brackets the kernels
+// of one class with HIP events on their launch stream; scopes on a stream that is being
+// captured into a graph record nothing.
+enum KClass : int {
+  KC_GEMM_PACKED = 0, KC_PACK = 1, KC_GEMM_MFMA = 2, KC_GEMM_VALU = 3,
+  KC_PERMUTE = 4, KC_WALK = 5, KC_GROUPED = 6, KC_OTHER = 7, KC_COUNT = 8
+};
+void timing_enable(bool on);
+bool timing_enabled();
+// Waits for the recorded events of class kc, returns their summed duration and count, and
+// clears them.
+void timing_collect(int kc, double* total_ms, int64_t* launches);
+struct TimedScope {
+  int kc;
+  hipStream_t st;
+  hipEvent_t start = nullptr;
+  TimedScope(int kc, hipStream_t st);
+  ~TimedScope();
+  TimedScope(const TimedScope&) = delete;
+  TimedScope& operator=(const TimedScope&) = delete;
+};
+
 enum Prec : int { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_FP64 = 2 };
 enum GemmKernel : int { GEMM_AUTO = -1, GEMM_VALU_TILE = 0, GEMM_VALU_DIRECT = 1, GEMM_MFMA = 2 };
 

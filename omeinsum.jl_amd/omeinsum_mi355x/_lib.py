@@ -14,7 +14,10 @@ OME_OK, OME_ERR_INVALID_ARG, OME_ERR_DIM_MISMATCH, OME_ERR_OOM = 0, 1, 2, 3
 OME_ERR_LOOPS_FORBIDDEN, OME_ERR_DEVICE, OME_ERR_UNSUPPORTED = 4, 5, 6
 F32, F64, C32, C64, I64, TF32, TF64 = range(7)
 OPT_PRECISION, OPT_COMPLEX_ALGO, OPT_ALLOW_LOOPS, OPT_DIRECT_GEMM = 0, 1, 2, 3
-OPT_USE_GRAPHS, OPT_AUTOTUNE, OPT_GEMM_KERNEL = 4, 5, 6
+OPT_USE_GRAPHS, OPT_AUTOTUNE, OPT_GEMM_KERNEL, OPT_KERNEL_TIMING = 4, 5, 6, 7
+# kernel classes of ome_get_kernel_timing
+KC = {"gemm_packed": 0, "pack": 1, "gemm_mfma": 2, "gemm_valu": 3, "permute": 4, "walk": 5,
+      "grouped": 6, "other": 7}
 PREC = {"fp32": 0, "bf16x3": 1, "fp64": 2}
 
 c_int, c_i32, c_i64, c_void_p, c_size_t, c_double, c_char = (
@@ -55,6 +58,7 @@ SIGNATURES = {
     "ome_get_option": (c_int, [c_void_p, c_int, P(c_i64)]),
     "ome_get_device_info": (c_int, [c_void_p, P(DeviceInfo)]),
     "ome_synchronize": (c_int, [c_void_p]),
+    "ome_get_kernel_timing": (c_int, [c_void_p, c_int, P(c_double), P(c_i64)]),
     "ome_alloc": (c_int, [c_void_p, c_size_t, P(c_void_p)]),
     "ome_free": (c_int, [c_void_p, c_void_p]),
     "ome_get_pool_stats": (c_int, [c_void_p, P(PoolStats)]),
@@ -176,6 +180,14 @@ class Context:
 
     def synchronize(self):
         check(self.lib.ome_synchronize(self.h), self.h)
+
+    def kernel_timing(self, kclass):
+        """(total milliseconds, launches) of the timed launches of one kernel class since the
+        last call (OPT_KERNEL_TIMING must be on while they ran)."""
+        ms, n = c_double(), c_i64()
+        check(self.lib.ome_get_kernel_timing(self.h, KC[kclass], ctypes.byref(ms),
+                                             ctypes.byref(n)), self.h)
+        return ms.value, n.value
 
 
 _contexts = {}
