@@ -223,7 +223,9 @@ int ome_grouped_gemm(ome_ctx* ctx, int dtype, int ngroups, const int64_t* dims,
  *          ref < 0 the internal node -(ref+1) (an earlier node).
  * slice_labels (may be NULL when nslice == 0) are the SlicedEinsum labels.  Intermediates
  * live in one device arena laid out by a static liveness plan; evaluation is recorded once
- * into a hipGraph and replayed. */
+ * into a hipGraph and replayed.  The context's planning options (precision, complex
+ * algorithm, allow_loops, direct GEMM, GEMM kernel) are frozen into the plan here; later
+ * changes of those options apply to plans created afterwards. */
 int ome_tree_create(ome_ctx* ctx, int dtype, const int32_t* desc, int64_t desc_len,
                     int nlabels, const int64_t* sizes, int nslice,
                     const int32_t* slice_labels, ome_tree** out);

@@ -134,6 +134,72 @@ __global__ __launch_bounds__(256) void k_gemm_direct(GemmDesc d, int order_m_fas
   }
 }
 
+// Few outputs and a long K (the last steps of a network contracted to a scalar: inner
+// products of two large tensors).  Block (output e, part p) of 256 threads sums k over its
+// chunk, strided by the block size, then reduces in LDS in a fixed tree order; with parts > 1
+// the partials go to work[p][e] and k_gemm_splitk_reduce finishes in part order.
+template <class O>
+__global__ __launch_bounds__(256) void k_gemm_dot(GemmDesc d, int64_t kchunk) {
+  using T = typename O::T;
+  __shared__ T red[256];
+  const int t = threadIdx.x;
+  const int64_t e = blockIdx.x, part = blockIdx.y;
+  const int64_t per = d.M * d.N;
+  const int64_t batch = e / per, r = e - batch * per;
+  const int64_t n = r / d.M, m = r - n * d.M;
+  const T* __restrict__ ap = (const T*)d.a + batch * d.sab + m * d.sam;
+  const T* __restrict__ bp = (const T*)d.b + batch * d.sbb + n * d.sbn;
+  const int64_t k0 = part * kchunk;
+  const int64_t k1 = (k0 + kchunk < d.K) ? k0 + kchunk : d.K;
+  T acc = O::zero();
+  for (int64_t k = k0 + t; k < k1; k += 256) acc = O::fma(ap[k * d.sak], bp[k * d.sbk], acc);
+  red[t] = acc;
+  __syncthreads();
+#pragma unroll
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) red[t] = O::add(red[t], red[t + s]);
+    __syncthreads();
+  }
+  if (t != 0) return;
+  if (d.ksplit > 1) {
+    ((T*)d.work)[part * per * d.B + e] = red[0];
+    return;
+  }
+  T* yp = (T*)d.y + batch * d.syb + m * d.sym + n * d.syn;
+  *yp = epilogue<O>(d.mode, O::from(d.sx), O::from(d.sy), red[0], yp);
+}
+
+int dot_plan_parts(const GemmDesc& d) {
+  const int64_t outputs = d.M * d.N * d.B;
+  int64_t parts = 1024 / (outputs > 0 ? outputs : 1);
+  const int64_t kmax = (d.K + 1023) / 1024;
+  if (parts > kmax) parts = kmax;
+  if (parts < 1) parts = 1;
+  if (parts > 4096) parts = 4096;
+  return (int)parts;
+}
+
+void launch_gemm_dot(int dt, GemmDesc d, int parts, hipStream_t st) {
+  const int64_t outputs = d.M * d.N * d.B;
+  if (outputs <= 0) return;
+  OME_DRY_RETURN(parts > 1 ? 2 : 1);
+  if (parts < 1) parts = 1;
+  const int64_t kchunk = (d.K + parts - 1) / parts;
+  d.ksplit = parts;
+  OME_DISPATCH(dt, O, {
+    d.mode = epi_mode<O>(d.sx, d.sy);
+    {
+      TimedScope ts(KC_GEMM_VALU, st);
+      hipLaunchKernelGGL(k_gemm_dot<O>, dim3((uint32_t)outputs, (uint32_t)parts), dim3(256), 0,
+                         st, d, kchunk);
+    }
+    if (parts > 1) {
+      TimedScope ts(KC_OTHER, st);
+      hipLaunchKernelGGL(k_gemm_splitk_reduce<O>, dim3(grid_cap(outputs)), dim3(256), 0, st, d);
+    }
+  });
+}
+
 int64_t gemm_valu_workspace(int dt, const GemmDesc& d, int ksplit) {
   return ksplit > 1 ? (int64_t)ksplit * d.B * d.M * d.N * (int64_t)elem_size(dt) : 0;
 }

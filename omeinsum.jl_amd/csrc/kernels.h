@@ -56,6 +56,25 @@ void launch_walk(int dt, const WalkDesc& d, hipStream_t st);
 int64_t gemm_valu_workspace(int dt, const GemmDesc& d, int ksplit);
 void launch_gemm_valu(int dt, GemmDesc d, int kernel, int ksplit, hipStream_t st);
 
+// Few outputs (M*N*batch <= 1024), long K: one workgroup per output and K part.
+int dot_plan_parts(const GemmDesc& d);
+void launch_gemm_dot(int dt, GemmDesc d, int parts, hipStream_t st);
+
+// GEMMs on the real matrix cores in another arithmetic than the element type: complex
+// products as real GEMMs (4M realified, 3M Gauss) and Float32 widened to Float64.
+// prep_a / prep_b write the real operands (sizes from prep_bytes), the real GEMM runs
+// on them, combine applies y <- sy*y + sx*C from the real result.
+enum RealAlgo : int { RA_WIDEN = 0, RA_4M = 1, RA_3M = 2 };
+struct RealShape {
+  int64_t M, N, K, B;  // of the real GEMM
+  int64_t a_elems, b_elems, c_elems;
+};
+RealShape real_shape(int algo, const GemmDesc& d);
+void launch_prep_a(int dt, bool rdouble, int algo, const GemmDesc& d, void* out, hipStream_t st);
+void launch_prep_b(int dt, bool rdouble, int algo, const GemmDesc& d, void* out, hipStream_t st);
+void launch_combine(int dt, bool rdouble, int algo, const GemmDesc& d, const void* c,
+                    hipStream_t st);
+
 int mfma_plan_ksplit(const GemmDesc& d, int requested);
 int64_t mfma_workspace(int dt, const GemmDesc& d, int ksplit);
 void launch_gemm_mfma(int prec, GemmDesc d, int ksplit, hipStream_t st);

@@ -241,6 +241,39 @@ static Status run_gemm_packed(ome_ctx* ctx, int dt, const GemmDesc& d, hipStream
   return Status::ok();
 }
 
+// Complex products as real GEMMs (4M / 3M) and Float32 widened to Float64: prepare dense
+// real operands, run the real GEMM (which picks its own kernel), combine into y.
+static Status run_gemm_via_real(ome_ctx* ctx, int dt, const GemmDesc& d, hipStream_t st,
+                                int precision, int algo) {
+  const bool rdouble = (dt == C64) || (precision == PREC_FP64);
+  const RealShape rs = real_shape(algo, d);
+  const size_t esz = rdouble ? 8 : 4;
+  Temp ta(ctx, (size_t)rs.a_elems * esz), tb(ctx, (size_t)rs.b_elems * esz);
+  Temp tc(ctx, (size_t)rs.c_elems * esz);
+  if (!ta.p || !tb.p || !tc.p)
+    return Status::err(OME_ERR_OOM, "device pool exhausted (real GEMM operands)");
+  launch_prep_a(dt, rdouble, algo, d, ta.p, st);
+  launch_prep_b(dt, rdouble, algo, d, tb.p, st);
+  GemmDesc g{};
+  g.a = ta.p;
+  g.b = tb.p;
+  g.y = tc.p;
+  g.M = rs.M;
+  g.N = rs.N;
+  g.K = rs.K;
+  g.B = rs.B;
+  g.sam = 1; g.sak = rs.M; g.sab = rs.M * rs.K;
+  g.sbk = 1; g.sbn = rs.K; g.sbb = rs.K * rs.N;
+  g.sym = 1; g.syn = rs.M; g.syb = rs.M * rs.N;
+  const int rdt = rdouble ? F64 : F32;
+  g.sx = scal_one(rdt);
+  g.sy = scal_zero(rdt);
+  Status s = run_gemm(ctx, rdt, g, st, rdouble ? PREC_FP64 : precision);
+  if (!s.good()) return s;
+  launch_combine(dt, rdouble, algo, d, tc.p, st);
+  return Status::ok();
+}
+
 Status run_gemm(ome_ctx* ctx, int dt, GemmDesc d, hipStream_t st, int precision) {
   if (d.M <= 0 || d.N <= 0 || d.B <= 0) return Status::ok();
   int kernel = ctx->opt.gemm_kernel;
@@ -249,6 +282,23 @@ Status run_gemm(ome_ctx* ctx, int dt, GemmDesc d, hipStream_t st, int precision)
     const double flops = 2.0 * d.M * d.N * d.K * d.B * (dt == C32 ? 4 : 1);
     if (d.M >= 64 && d.N >= 64 && d.K >= 64 && flops >= 3.0e7)
       return run_gemm_packed(ctx, dt, d, st);
+  }
+  if (kernel == GEMM_AUTO || kernel == GEMM_MFMA) {
+    const bool fits = d.M * d.K * d.B < (1ll << 31) && d.K * d.N * d.B < (1ll << 31) &&
+                      d.M * d.N * d.B < (1ll << 31) && d.M < (1ll << 31) && d.N < (1ll << 31) &&
+                      d.K < (1ll << 31);
+    // Float32 at FP64 precision: widened operands on the FP64 matrix cores
+    if (dt == F32 && precision == PREC_FP64 && fits)
+      return run_gemm_via_real(ctx, dt, d, st, precision, RA_WIDEN);
+    // complex: real GEMMs on the matrix cores once the product is large enough to pay for
+    // the operand preparation (otherwise the complex VALU kernels below)
+    const bool cx = (dt == C32 || dt == C64);
+    const int algo = ctx->opt.complex_algo ? RA_3M : RA_4M;
+    const int64_t rm = d.M, rn = algo == RA_4M ? 2 * d.N : d.N, rk = algo == RA_4M ? 2 * d.K : d.K;
+    if (cx && fits && rm >= 32 && rn >= 32 && rk >= 16 && d.M * d.N * d.K * d.B >= (1ll << 16))
+      return run_gemm_via_real(ctx, dt, d, st, precision, algo);
+    if (dt == C32 && precision == PREC_FP64 && fits)
+      return run_gemm_via_real(ctx, dt, d, st, precision, algo);
   }
   const bool mfma_type = (dt == F32 || dt == F64);
   const bool a_unit = d.sam == 1 || d.sak == 1 || d.M == 1 || d.K == 1;
@@ -276,6 +326,18 @@ Status run_gemm(ome_ctx* ctx, int dt, GemmDesc d, hipStream_t st, int precision)
       d.work = ws.p;
     }
     launch_gemm_mfma(prec, d, ks, st);
+    return Status::ok();
+  }
+  // few outputs and a long K: parallel over K instead of one thread (or one tile) per output
+  if (d.M * d.N * d.B <= 1024 && d.K >= 2048 && ctx->opt.gemm_kernel == GEMM_AUTO) {
+    const int parts = dot_plan_parts(d);
+    Temp ws;
+    if (parts > 1) {
+      ws = Temp(ctx, (size_t)parts * d.M * d.N * d.B * elem_size(dt));
+      if (!ws.p) return Status::err(OME_ERR_OOM, "device pool exhausted (split-K workspace)");
+      d.work = ws.p;
+    }
+    launch_gemm_dot(dt, d, parts, st);
     return Status::ok();
   }
   int ks = 1;

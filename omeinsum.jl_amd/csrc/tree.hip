@@ -123,12 +123,29 @@ struct ome_tree {
   ome_tree_info info{};
   std::list<GraphEntry> graphs;  // most recent first
   hipStream_t cap_stream = nullptr;
+  ome::Options opt;  // planning options frozen at creation
   // gradient plan (lazily built)
   char* gmem = nullptr;
   int64_t g_arena = 0, g_scratch = 0;
 };
 
 namespace {
+
+// The planning options of a tree are frozen when it is created: its scratch is sized, its
+// graphs recorded and every evaluation planned with the same kernel choices, whatever the
+// context's options are at the time of a later call.
+struct OptScope {
+  ome_ctx* ctx;
+  ome::Options saved;
+  OptScope(ome_ctx* c, const ome::Options& o) : ctx(c), saved(c->opt) {
+    ctx->opt.precision = o.precision;
+    ctx->opt.complex_algo = o.complex_algo;
+    ctx->opt.allow_loops = o.allow_loops;
+    ctx->opt.direct_gemm = o.direct_gemm;
+    ctx->opt.gemm_kernel = o.gemm_kernel;
+  }
+  ~OptScope() { ctx->opt = saved; }
+};
 
 int64_t prod(const std::vector<int>& ls, const std::vector<int64_t>& sz) {
   int64_t p = 1;
@@ -337,6 +354,7 @@ int ome_tree_create(ome_ctx* ctx, int dtype, const int32_t* desc, int64_t desc_l
   ome_tree* t = new ome_tree();
   t->ctx = ctx;
   t->dt = dtype;
+  t->opt = ctx->opt;
   t->sizes.assign(sizes, sizes + nlabels);
   for (int64_t s : t->sizes)
     if (s < 0) { delete t; return tree_fail(ctx, Status::err(OME_ERR_INVALID_ARG, "negative size")); }
@@ -513,6 +531,7 @@ int ome_tree_execute(ome_tree* t, const void* const* xs, void* y, const ome_scal
   if (!y) return tree_fail(t->ctx, Status::err(OME_ERR_INVALID_ARG, "null output pointer"));
   ome_ctx* ctx = t->ctx;
   std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  OptScope os(ctx, t->opt);
   hipSetDevice(ctx->device);
   const Scal sx = sx_ ? Scal{sx_->re, sx_->im, sx_->i} : scal_one(t->dt);
   const Scal sy = sy_ ? Scal{sy_->re, sy_->im, sy_->i} : scal_zero(t->dt);
@@ -580,6 +599,7 @@ int ome_tree_execute_scalar_normalized(ome_tree* t, const void* const* xs, doubl
     return tree_fail(ctx, Status::err(OME_ERR_INVALID_ARG,
                                       "normalised evaluation needs a float/complex scalar output"));
   std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  OptScope os(ctx, t->opt);
   hipSetDevice(ctx->device);
   hipStream_t st = ctx->stream;
   std::vector<const void*> xv(xs, xs + t->n_leaves);
@@ -633,6 +653,7 @@ int ome_tree_gradient(ome_tree* t, const void* const* xs, const void* ybar, void
     return tree_fail(t ? t->ctx : nullptr, Status::err(OME_ERR_INVALID_ARG, "null argument"));
   ome_ctx* ctx = t->ctx;
   std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  OptScope os(ctx, t->opt);
   hipSetDevice(ctx->device);
   hipStream_t st = ctx->stream;
   const size_t es = elem_size(t->dt);

@@ -1,7 +1,8 @@
 """Device plans for contraction trees (NestedEinsum / SlicedEinsum) over ome_tree_*.
 
-A plan is compiled once per (tree, label sizes, element type) and cached; evaluation replays
-a recorded hipGraph when the same input/output buffers come back (see csrc/tree.hip).
+A plan is compiled once per (tree, label sizes, element type, planning options) and cached;
+evaluation replays a recorded hipGraph when the same input/output buffers come back (see
+csrc/tree.hip).
 """
 import collections
 import ctypes
@@ -150,7 +151,10 @@ def plan_for(code, xs, size_dict):
         raise TypeError("tree inputs must share one element type (promote first)")
     code_dt = dtype_code(T, trop)
     labels = code.uniquelabels() if not isinstance(code, SlicedEinsum) else code.eins.uniquelabels()
-    key = (code, tuple((l, size_dict[l]) for l in labels), code_dt, ts[0].device.index)
+    # a plan freezes the context's planning options when it is created
+    ctx = _lib.context(ts[0].device.index)
+    opts = tuple(ctx.get_option(k) for k in _lib.PLAN_OPTIONS)
+    key = (code, tuple((l, size_dict[l]) for l in labels), code_dt, ts[0].device.index, opts)
     plan = _CACHE.get(key)
     if plan is None:
         plan = TreePlan(code, size_dict, code_dt, ts[0].device.index)

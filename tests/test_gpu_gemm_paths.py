@@ -1,0 +1,138 @@
+"""GPU parity of the GEMM routes the planner picks by shape and precision, through the C ABI
+against the oracle: complex products as real matrix-core GEMMs (4M realified, 3M Gauss),
+Float32 at FP64 precision (widened operands, one rounding), and the few-outputs / long-K
+inner products."""
+import numpy as np
+import pytest
+
+from helpers import assert_close, parse, rand, rng, wide
+from oracle import loop
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    yield
+    import omeinsum_mi355x as om
+    om.set_einsum_backend(om.DefaultBackend())
+
+
+def om():
+    import omeinsum_mi355x
+    return omeinsum_mi355x
+
+
+def dev(x, tropical=False):
+    t = torch.from_numpy(np.array(x, order="C")).cuda()
+    return om().Tropical(t) if tropical else t
+
+
+def run(ixs, iy, xs, sd, backend, y0=None, sx=True, sy=False, tropical=False):
+    o = om()
+    o.set_einsum_backend(backend)
+    try:
+        code = o.EinCode(ixs, iy)
+        dxs = [dev(x, tropical) for x in xs]
+        if y0 is None:
+            y = o.einsum(code, dxs, sd)
+        else:
+            y = dev(y0)
+            o.einsum_inplace(code, dxs, y, sx, sy, sd)
+        torch.cuda.synchronize()
+        return o.to_numpy(y)
+    finally:
+        o.set_einsum_backend(o.DefaultBackend())
+
+
+def oracle(ixs, iy, xs, sd, y0=None, sx=True, sy=False, tropical=False):
+    T = np.result_type(*[x.dtype for x in xs])
+    W = T if tropical else wide(T)
+    y = None if y0 is None else np.asarray(y0, dtype=W)
+    return loop.loop_einsum(ixs, iy, [np.asarray(x, dtype=W) for x in xs], sd, y=y, sx=sx,
+                            sy=sy, tropical=tropical)
+
+
+SPECS = ["ij,jk->ik", "ji,kj->ik", "ijb,jkb->ikb", "ajb,bkj->bka", "ij,jk->ki"]
+SIZES = {"i": 70, "j": 90, "k": 65, "a": 33, "b": 3}
+
+
+@pytest.mark.parametrize("spec", SPECS)
+@pytest.mark.parametrize("algo", ["4m", "3m"])
+@pytest.mark.parametrize("dtype,prec", [(np.complex64, "fp32"), (np.complex64, "bf16x3"),
+                                        (np.complex64, "fp64"), (np.complex128, "fp32")])
+def test_complex_as_real_gemm(spec, algo, dtype, prec):
+    # M*N*K >= 2^16 with M, 2N >= 32: the planner takes the real-GEMM route
+    ixs, iy = parse(spec)
+    r = rng(2101)
+    xs = [rand(r, [SIZES[l] for l in ix], dtype) for ix in ixs]
+    y0 = rand(r, [SIZES[l] for l in iy], dtype)
+    be = om().MI355XBackend(precision=prec, complex_algo=algo)
+    got = run(ixs, iy, xs, SIZES, be)
+    ref = oracle(ixs, iy, xs, SIZES)
+    k = SIZES["j"]
+    # 3M forms (ar+ai)(br+bi) - ar*br - ai*bi: operands up to twice as large, three terms
+    assert_close(got, ref, dtype, 4 * k * (3 if algo == "3m" else 1), spec)
+    sx, sy = 1.5 - 0.5j, 0.25 + 1j
+    got = run(ixs, iy, xs, SIZES, be, y0=y0, sx=sx, sy=sy)
+    ref = oracle(ixs, iy, xs, SIZES, y0=y0, sx=sx, sy=sy)
+    assert_close(got, ref, dtype, 16 * k * (3 if algo == "3m" else 1), spec + " scaled")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
+def test_fp64_precision_rounds_once(dtype):
+    # precision="fp64": Float32 / ComplexF32 operands are widened and multiplied on the FP64
+    # matrix cores, so every output component is the FP64 result rounded once to Float32:
+    # |got - exact| <= 2^-24 |exact| (plus the FP64 reference's own error).
+    r = rng(2102)
+    ixs, iy = parse("ij,jk->ik")
+    sd = {"i": 200, "j": 300, "k": 150}
+    xs = [rand(r, [sd[l] for l in ix], dtype) for ix in ixs]
+    got = run(ixs, iy, xs, sd, om().MI355XBackend(precision="fp64"))
+    ref = oracle(ixs, iy, xs, sd)
+    scale = np.max(np.abs(ref))
+    parts = [(np.real, np.real)] + ([(np.imag, np.imag)] if dtype == np.complex64 else [])
+    for f, g in parts:
+        e = np.abs(f(got).astype(np.float64) - g(ref))
+        assert np.all(e <= 2.0 ** -24 * np.abs(g(ref)) + 1e-13 * scale), e.max()
+    # the FP32 matrix-core product misses that bound somewhere (the check has teeth)
+    got32 = run(ixs, iy, xs, sd, om().MI355XBackend(precision="fp32"))
+    e32 = np.abs(np.real(got32).astype(np.float64) - np.real(ref))
+    assert np.any(e32 > 2.0 ** -24 * np.abs(np.real(ref)) + 1e-13 * scale)
+
+
+LONG_K = [("ij,ij->", {"i": 256, "j": 300}),            # one output, K = 76800
+          ("ijk,jkl->il", {"i": 3, "j": 100, "k": 100, "l": 5}),  # 15 outputs, K = 10^4
+          ("abc,abd->cd", {"a": 64, "b": 64, "c": 2, "d": 3})]    # 6 outputs, K = 4096
+
+
+@pytest.mark.parametrize("spec,sd", LONG_K)
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.complex64, np.int64, "tropical"])
+def test_few_outputs_long_k(spec, sd, dtype):
+    trop = dtype == "tropical"
+    dt = np.float64 if trop else dtype
+    ixs, iy = parse(spec)
+    r = rng(2103)
+    xs = [rand(r, [sd[l] for l in ix], dt) for ix in ixs]
+    if trop:
+        xs = [np.round(x * 8) for x in xs]  # exact sums: bit-exact max-plus
+    for be in (om().DefaultBackend(), om().MI355XBackend()):
+        got = run(ixs, iy, xs, sd, be, tropical=trop)
+        ref = oracle(ixs, iy, xs, sd, tropical=trop)
+        if trop or dt == np.int64:
+            assert np.array_equal(got, ref), spec
+        else:
+            k = int(np.prod([sd[l] for l in set(ixs[0]) & set(ixs[1]) - set(iy)]))
+            assert_close(got, ref, dt, k, spec)
+        y0 = rand(r, [sd[l] for l in iy], dt)
+        if not trop:
+            sx, sy = (3, -2) if dt == np.int64 else (0.5, 2.0)
+            got = run(ixs, iy, xs, sd, be, y0=y0, sx=sx, sy=sy)
+            ref = oracle(ixs, iy, xs, sd, y0=y0, sx=sx, sy=sy)
+            if dt == np.int64:
+                assert np.array_equal(got, ref), spec
+            else:
+                assert_close(got, ref, dt, 4 * k, spec + " scaled")

@@ -195,3 +195,29 @@ def test_gradient_larger_tree_and_petersen():
     ptree = co.optimize_code(pc, codes.uniformsize(pc, 3))
     cost, grads = o.cost_and_gradient(ptree, [dev(s)] * 10)
     assert o.to_numpy(cost)[()] == 0 and o.to_numpy(grads[0]).sum() == 0
+
+
+def test_plan_freezes_planning_options():
+    # a plan keeps the precision it was created with; the backend's precision selects the plan
+    o = om()
+    from omeinsum_mi355x import _lib
+    from omeinsum_mi355x.tree import plan_for
+    r = rng(2006)
+    ne = codes.ein("(ij,jk),kl->il")
+    sd = dict(zip("ijkl", (256, 256, 256, 256)))
+    xs = [rand(r, (256, 256), np.float32) for _ in range(3)]
+    dxs = [dev(x) for x in xs]
+    ref = db.nested_einsum(ne.to_plain(), [x.astype(np.float64) for x in xs], sd)
+    use("mi355x", precision="fp32")
+    r32 = o.to_numpy(ne(*dxs))
+    plan32 = plan_for(ne, dxs, sd)
+    use("mi355x", precision="bf16x3")
+    r3 = o.to_numpy(ne(*dxs))
+    assert_close(r32, ref, np.float32, 256 * 256)
+    assert_close(r3, ref, np.float32, 256 * 256)
+    assert not np.array_equal(r32, r3)  # two plans, two arithmetics
+    # the fp32 plan, run while the context says bf16x3, still computes in fp32
+    ctx = _lib.context()
+    assert ctx.get_option(_lib.OPT_PRECISION) == _lib.PREC["bf16x3"]
+    again = o.to_numpy(plan32.execute(dxs))
+    assert np.array_equal(again, r32)
