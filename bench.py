@@ -14,10 +14,11 @@ the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this kernel, when pre
 
 Also reported in the same line: the same contraction in plain FP32 MFMA and its error vs
 FP64; ijk,jkl->il at 64^3; the 100-vertex 3-regular ComplexF32 network contracted to a scalar
-by the MI355X backend (graph replay) next to the plain per-step GPU path; the CPU baseline
-(the oracle's numpy restatement of OMEinsum's DefaultBackend on this host).
+by the MI355X backend (graph replay) next to the plain per-step GPU path; 4096 uneven batched
+GEMMs and 1024 permutes in two grouped launches next to one call per problem; the CPU
+baseline (the oracle's numpy restatement of OMEinsum's DefaultBackend on this host).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|c4]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|c4|c3]
                        [--no-cpu] [--copy-calibration]
 """
 import argparse
@@ -189,6 +190,96 @@ def bench_c2(steps, warmup):
     return out
 
 
+def bench_c3(steps, warmup):
+    """4096 uneven batched GEMMs (ein"ijb,jkb->ikb", i, j, k in {8, 16, ..., 256}, b in 1..16,
+    Float32) in one grouped launch, plus 1024 ein"ijkl->ljik" permutes (dims in [4, 64]) in
+    one grouped launch; against the per-problem path (one library call per GEMM / permute)."""
+    import numpy as np
+    import torch
+    import omeinsum_mi355x as om
+    from omeinsum_mi355x import _lib
+
+    r = np.random.Generator(np.random.PCG64(3000))
+    g = torch.Generator(device="cuda").manual_seed(3001)
+    dims8 = np.arange(8, 257, 8)
+    probs, flops, gemm_bytes = [], 0.0, 0.0
+    for _ in range(4096):
+        i, j, k = (int(x) for x in r.choice(dims8, 3))
+        b = int(r.integers(1, 17))
+        # column-major A[i,j,b], B[j,k,b], C[i,k,b] are row-major (b,j,i), (b,k,j), (b,k,i):
+        # C_t = B_t @ A_t in torch's batch-major convention
+        At = torch.rand(b, j, i, device="cuda", generator=g) * 2 - 1
+        Bt = torch.rand(b, k, j, device="cuda", generator=g) * 2 - 1
+        Ct = torch.empty(b, k, i, device="cuda")
+        probs.append((Bt, At, Ct))
+        flops += 2.0 * i * j * k * b
+        gemm_bytes += 4.0 * b * (i * j + j * k + i * k)
+    xs, ys, perm_bytes = [], [], 0.0
+    for _ in range(1024):
+        i, j, k, l = (int(x) for x in r.integers(4, 65, 4))
+        # column-major x[i,j,k,l] -> y[l,j,i,k] is row-major (l,k,j,i) -> (k,i,j,l)
+        x = torch.rand(l, k, j, i, device="cuda", generator=g)
+        xs.append(x)
+        ys.append(torch.empty(k, i, j, l, device="cuda"))
+        perm_bytes += 8.0 * i * j * k * l
+    perms = [(1, 3, 2, 0)] * len(xs)
+    gg = om.GroupedGemm(probs)
+    gp = om.GroupedPermute(xs, perms, ys)
+    ctx = _lib.context()
+    out = {"instances": 4096, "permutes": 1024, "gemm_flop": flops, "gemm_bytes": gemm_bytes,
+           "permute_bytes": perm_bytes}
+    names = {0: "f32 64x64", 1: "f32 64x64 bk32", 2: "f32 128x128", 3: "bf16x3 64x64",
+             4: "bf16x3 128x128", 5: "f32 wave 32x32", 6: "bf16x3 wave 32x32"}
+    for prec in ("fp32", "bf16x3"):
+        ctx.set_option(_lib.OPT_PRECISION, _lib.PREC[prec])
+        gg()  # first call: autotune the tile variant and upload the descriptors
+        gp()
+        torch.cuda.synchronize()
+        choice = ctx.get_option(_lib.OPT_LAST_GROUPED_CHOICE)
+        ctx.set_option(_lib.OPT_KERNEL_TIMING, 1)
+        try:
+            ms = time_events(lambda: (gg(), gp()), steps, warmup,
+                             after_warmup=lambda: [ctx.kernel_timing(k) for k in ("grouped", "permute")])
+            g_ms, g_n = ctx.kernel_timing("grouped")
+            p_ms, p_n = ctx.kernel_timing("permute")
+        finally:
+            ctx.set_option(_lib.OPT_KERNEL_TIMING, 0)
+        gk, pk = g_ms / max(g_n, 1), p_ms / max(p_n, 1)
+        # accuracy on a sample of instances against FP64
+        errs = []
+        for q in range(0, 4096, 256):
+            Bt, At, Ct = probs[q]
+            ref = torch.matmul(Bt.double(), At.double())
+            errs.append(float((Ct.double() - ref).norm() / ref.norm()))
+        out[prec] = {"ms_per_step": ms, "grouped_gemm_us": gk * 1e3,
+                     "grouped_permute_us": pk * 1e3,
+                     "tile": names.get(choice // 2, "?") + (" persistent" if choice % 2 else ""),
+                     "gemm_tflops": flops / gk / 1e9,
+                     "gemm_frac_peak": flops / gk / 1e9 / (PEAK_FP32 if prec == "fp32" else PEAK_BF16 / 3),
+                     "gemm_gbs": gemm_bytes / gk / 1e6, "gemm_frac_hbm": gemm_bytes / gk / 1e6 / PEAK_HBM,
+                     "permute_gbs": perm_bytes / pk / 1e6,
+                     "permute_frac_hbm": perm_bytes / pk / 1e6 / PEAK_HBM,
+                     "max_rel_err_fp64": max(errs)}
+        log("C3 %s grouped: %.3f ms/step (GEMM kernel %.1f us = %.1f TFLOP/s, %s; permute kernel "
+            "%.1f us = %.0f GB/s); max rel err %.1e" % (prec, ms, gk * 1e3, flops / gk / 1e9,
+                                                         out[prec]["tile"], pk * 1e3,
+                                                         perm_bytes / pk / 1e6, max(errs)))
+    ctx.set_option(_lib.OPT_PRECISION, 0)
+    for y, x in zip(ys[:64], xs[:64]):
+        assert torch.equal(y, x.permute(1, 3, 2, 0))
+    # the per-problem path: one library call per GEMM and per permute
+    def per_problem():
+        for Bt, At, Ct in probs:
+            om.batched_gemm("N", "N", True, Bt, At, False, Ct)
+        for x, y in zip(xs, ys):
+            om.tensorpermute(y, x, (1, 3, 2, 0))
+    out["per_problem_fp32_ms"] = time_events(per_problem, 2, 1)
+    log("C3 per-problem path (one call per GEMM / permute): %.2f ms" % out["per_problem_fp32_ms"])
+    del probs, xs, ys, gg, gp
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_calibration(reps=5):
     """A streaming copy of 1 GiB through one elementwise kernel (y = x + 0: each element read
     once, written once), repeated: the reading of the FETCH_SIZE / WRITE_SIZE counters for a
@@ -244,7 +335,7 @@ def main():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", choices=("all", "c4"), default="all")
+    p.add_argument("--workload", choices=("all", "c4", "c3"), default="all")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--copy-calibration", action="store_true")
     args = p.parse_args()
@@ -258,12 +349,14 @@ def main():
         sys.exit(1)
     import omeinsum_mi355x as om  # noqa: F401
 
-    precs = ("bf16x3",) if args.workload == "c4" else ("bf16x3", "fp32")
+    precs = ("bf16x3", "fp32") if args.workload == "all" else ("bf16x3",)
     c4, ab = bench_c4(args.steps, args.warmup, precs)
     extra = {"c4": c4}
     if args.workload == "all":
         extra["c1_ijk_jkl_il_64"] = bench_c1(args.steps, args.warmup)
         extra["c2_network_100"] = bench_c2(max(5, args.steps // 2), 2)
+    if args.workload in ("all", "c3"):
+        extra["c3_grouped"] = bench_c3(max(3, args.steps // 4), 1)
     if args.copy_calibration:
         copy_calibration()
     cpu = None if args.no_cpu else cpu_baseline(ab)

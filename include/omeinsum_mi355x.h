@@ -81,7 +81,13 @@ typedef enum ome_option {
   OME_OPT_USE_GRAPHS = 4,   /* 1 (default): tree plans replay a captured hipGraph */
   OME_OPT_AUTOTUNE = 5,     /* 1 (default): time GEMM tile candidates on first use of a shape class */
   OME_OPT_GEMM_KERNEL = 6,  /* force a GEMM kernel family (-1 = automatic; for tests) */
-  OME_OPT_KERNEL_TIMING = 7 /* 1: bracket kernel launches with HIP events (process-wide) */
+  OME_OPT_KERNEL_TIMING = 7, /* 1: bracket kernel launches with HIP events (process-wide) */
+  OME_OPT_GROUPED_VARIANT = 8, /* grouped GEMM tile variant: -1 autotuned (default), 0 64x64,
+                                  1 64x64 with K step 32, 2 128x128, 3 BF16x3 64x64,
+                                  4 BF16x3 128x128, 5 one wave per 32x32 tile,
+                                  6 BF16x3 one wave per 32x32 tile (Float32 only; for tests) */
+  OME_OPT_LAST_GROUPED_CHOICE = 9 /* read-only: 2*variant + persistent of the last grouped
+                                     GEMM launch (the autotuner's pick) */
 } ome_option;
 
 /* Kernel classes of ome_get_kernel_timing. */
@@ -206,12 +212,22 @@ int ome_batched_gemm(ome_ctx* ctx, int dtype, char ta, char tb, int64_t m, int64
                      const void* b, const ome_scalar* beta, void* c);
 
 /* No reference counterpart: many independent batched GEMMs of uneven shapes in one
- * persistent launch (the shallow levels of contraction trees, independent networks).
+ * launch (the shallow levels of contraction trees, independent networks).
  * Each problem g: C_g[m,n,l] = alpha*sum_k A_g[m,k,l] B_g[k,n,l] + beta*C_g, column-major,
- * no transposes; dims holds (m,n,k,batch) per problem. */
+ * dense, no transposes; dims holds (m,n,k,batch) per problem.  Float32 runs on the FP32
+ * matrix cores, other types on the semiring VALU tile.  The tile shape and grid are chosen
+ * per shape class by timing the candidates on first use (when beta is zero, autotuning is
+ * on and the stream is not being captured) and remembered for the context. */
 int ome_grouped_gemm(ome_ctx* ctx, int dtype, int ngroups, const int64_t* dims,
                      const void* const* a, const void* const* b, void* const* c,
                      const ome_scalar* alpha, const ome_scalar* beta);
+
+/* No reference counterpart: many tensorpermute! calls (src/utils.jl:122-156) in one launch.
+ * Problem g permutes a[g] (ndims[g] dims; extents in dims, concatenated over problems) into
+ * c[g]: c dim p = a dim perms[p] (0-based, concatenated), c <- sy*c + sx*permutedims(a). */
+int ome_grouped_permute(ome_ctx* ctx, int dtype, int nperm, const int32_t* ndims,
+                        const int64_t* dims, const int32_t* perms, const void* const* a,
+                        void* const* c, const ome_scalar* sx, const ome_scalar* sy);
 
 /* ---- contraction trees -------------------------------------------------------------- */
 

@@ -242,6 +242,7 @@ int ome_shutdown(ome_ctx* ctx) {
   OME_CHECK_CTX(ctx);
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  destroy_group_state(ctx);
   delete ctx->pool;
   delete ctx;
   return OME_OK;
@@ -272,6 +273,9 @@ int ome_set_option(ome_ctx* ctx, int key, int64_t v) {
       if (v < -1 || v > 2) return fail(ctx, OME_ERR_INVALID_ARG, "gemm kernel must be -1..2");
       ctx->opt.gemm_kernel = (int)v; break;
     case OME_OPT_KERNEL_TIMING: timing_enable(v != 0); break;
+    case OME_OPT_GROUPED_VARIANT:
+      if (v < -1 || v >= GG_NVARIANTS) return fail(ctx, OME_ERR_INVALID_ARG, "grouped variant must be -1..6");
+      ctx->opt.grouped_variant = (int)v; break;
     default: return fail(ctx, OME_ERR_INVALID_ARG, "unknown option");
   }
   return OME_OK;
@@ -289,6 +293,8 @@ int ome_get_option(ome_ctx* ctx, int key, int64_t* v) {
     case OME_OPT_AUTOTUNE: *v = ctx->opt.autotune; break;
     case OME_OPT_GEMM_KERNEL: *v = ctx->opt.gemm_kernel; break;
     case OME_OPT_KERNEL_TIMING: *v = timing_enabled() ? 1 : 0; break;
+    case OME_OPT_GROUPED_VARIANT: *v = ctx->opt.grouped_variant; break;
+    case OME_OPT_LAST_GROUPED_CHOICE: *v = ctx->last_grouped_choice; break;
     default: return fail(ctx, OME_ERR_INVALID_ARG, "unknown option");
   }
   return OME_OK;

@@ -1,0 +1,710 @@
+// Grouped launches: many independent problems of uneven shapes in one persistent launch.
+//
+// Grouped GEMM: problem g computes C_g[:,:,l] = alpha*A_g[:,:,l]*B_g[:,:,l] + beta*C_g[:,:,l]
+// (column-major, dense, no transposes; A m x k, B k x n).  Every problem is cut into output
+// tiles; the tiles of all problems form one list, ordered by problem (the host sorts the
+// problems by K, longest first, so tiles of similar cost are adjacent).  A fixed grid of
+// workgroups walks the list round-robin (tile = blockIdx.x + i*gridDim.x) and finds the
+// problem of a tile in a tile -> problem table (one load, not a search).
+//
+//   Float32:          MFMA v_mfma_f32_32x32x2_f32, workgroup tile TM x TN (4 waves, 2x2),
+//                     K step BK staged through LDS with a register prefetch of the next step;
+//                     at BF16x3 precision the same tiles split into bf16 hi/lo while staged
+//                     and run on v_mfma_f32_32x32x16_bf16.
+//   other types:      semiring VALU tile 64 x 64 (4 x 4 outputs per thread), as in the
+//                     strided GEMM (exact Int64, max-plus, complex).
+//
+// Grouped permute: problem g writes y_g = sx*permutedims(x_g, perm_g) + sy*y_g.  Each problem
+// is a list of 64 x 64 LDS-transposed tiles (when the input's contiguous dim moves) or of
+// 1024-element runs along the output's fastest dim; the blocks of all problems form one grid.
+#include "kernels.h"
+
+namespace ome {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// ---- grouped GEMM, Float32 on the matrix cores -------------------------------------------
+
+template <int TM, int TN, int BK>
+__global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g) {
+  constexpr int LDK = BK + 4;
+  constexpr int MB = TM / 64, NB = TN / 64;       // 32x32 blocks per wave along m / n
+  constexpr int NA = TM * BK / 1024;              // float4 loads per thread, A tile
+  constexpr int NBL = TN * BK / 1024;             // float4 loads per thread, B tile
+  static_assert(NA >= 1 && NBL >= 1, "tile too small for 256 threads");
+  __shared__ __attribute__((aligned(16))) float sA[TM][LDK];
+  __shared__ __attribute__((aligned(16))) float sB[TN][LDK];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
+  const float alpha = (float)g.alpha.re, beta = (float)g.beta.re;
+  for (int64_t tile = blockIdx.x; tile < g.total_tiles; tile += gridDim.x) {
+    const int p = g.map[tile];
+    const GGProb pr = g.probs[p];
+    const int64_t r = tile - g.first[p];
+    const int64_t per = (int64_t)pr.tiles_m * pr.tiles_n;
+    const int64_t batch = r / per;
+    const int64_t rr = r - batch * per;
+    const int64_t m0 = (rr % pr.tiles_m) * TM, n0 = (rr / pr.tiles_m) * TN;
+    const int64_t M = pr.m, N = pr.n, K = pr.k;
+    const float* __restrict__ A = (const float*)pr.a + batch * M * K;
+    const float* __restrict__ B = (const float*)pr.b + batch * K * N;
+    floatx16 acc[NB][MB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < MB; ++j)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.f;
+    // A tile (m fastest in memory): load q covers m4 = (e % (TM/4))*4, k = e / (TM/4)
+    // B tile (k fastest in memory): load q covers k4 = (e % (BK/4))*4, n = e / (BK/4)
+    float4 ra[NA], rb[NBL];
+    auto load = [&](int64_t k0) __attribute__((always_inline)) {
+#pragma unroll
+      for (int q = 0; q < NA; ++q) {
+        const int e = t + 256 * q;
+        const int64_t m = m0 + (e % (TM / 4)) * 4, k = k0 + e / (TM / 4);
+        const float* src = A + m + k * M;
+        if (pr.vec_a && m + 4 <= M && k < K) {
+          ra[q] = *(const float4*)src;
+        } else {
+          ra[q].x = (m < M && k < K) ? src[0] : 0.f;
+          ra[q].y = (m + 1 < M && k < K) ? src[1] : 0.f;
+          ra[q].z = (m + 2 < M && k < K) ? src[2] : 0.f;
+          ra[q].w = (m + 3 < M && k < K) ? src[3] : 0.f;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NBL; ++q) {
+        const int e = t + 256 * q;
+        const int64_t k = k0 + (e % (BK / 4)) * 4, n = n0 + e / (BK / 4);
+        const float* src = B + k + n * K;
+        if (pr.vec_b && k + 4 <= K && n < N) {
+          rb[q] = *(const float4*)src;
+        } else {
+          rb[q].x = (k < K && n < N) ? src[0] : 0.f;
+          rb[q].y = (k + 1 < K && n < N) ? src[1] : 0.f;
+          rb[q].z = (k + 2 < K && n < N) ? src[2] : 0.f;
+          rb[q].w = (k + 3 < K && n < N) ? src[3] : 0.f;
+        }
+      }
+    };
+    load(0);
+    for (int64_t k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+      for (int q = 0; q < NA; ++q) {
+        const int e = t + 256 * q;
+        const int m = (e % (TM / 4)) * 4, k = e / (TM / 4);
+        sA[m][k] = ra[q].x;
+        sA[m + 1][k] = ra[q].y;
+        sA[m + 2][k] = ra[q].z;
+        sA[m + 3][k] = ra[q].w;
+      }
+#pragma unroll
+      for (int q = 0; q < NBL; ++q) {
+        const int e = t + 256 * q;
+        const int k = (e % (BK / 4)) * 4, n = e / (BK / 4);
+        *(float4*)&sB[n][k] = rb[q];
+      }
+      __syncthreads();
+      if (k0 + BK < K) load(k0 + BK);
+      // lanes 0-31 take k = kb + s, lanes 32-63 k = kb + 4 + s in the s-th MFMA of a group;
+      // a wave whose whole block lies outside the problem skips its MFMAs
+      if (m0 + wm < M && n0 + wn < N) {
+#pragma unroll
+        for (int kb = 0; kb < BK; kb += 8) {
+          const int kk = kb + 4 * (lane >> 5);
+          float4 af[NB], bf[MB];
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb) af[nb] = *(const float4*)&sB[wn + nb * 32 + (lane & 31)][kk];
+#pragma unroll
+          for (int mb = 0; mb < MB; ++mb) bf[mb] = *(const float4*)&sA[wm + mb * 32 + (lane & 31)][kk];
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+              acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[nb].x, bf[mb].x, acc[nb][mb], 0, 0, 0);
+              acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[nb].y, bf[mb].y, acc[nb][mb], 0, 0, 0);
+              acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[nb].z, bf[mb].z, acc[nb][mb], 0, 0, 0);
+              acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[nb].w, bf[mb].w, acc[nb][mb], 0, 0, 0);
+            }
+        }
+      }
+      __syncthreads();
+    }
+    // lane holds m = m0 + wm + mb*32 + (lane & 31), n = n0 + wn + nb*32 + 8*(q/4) + 4*(lane/32) + q%4
+    float* __restrict__ C = (float*)pr.c + batch * M * N;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int64_t m = m0 + wm + mb * 32 + (lane & 31);
+          const int64_t n = n0 + wn + nb * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+          if (m < M && n < N) {
+            float* yp = C + m + n * M;
+            *yp = epilogue<OpsF32>(g.mode, alpha, beta, acc[nb][mb][q], yp);
+          }
+        }
+  }
+}
+
+// ---- grouped GEMM, Float32 as BF16x3 on the matrix cores ---------------------------------
+// Operands are split while they are staged: x = hi + lo (bf16, round to nearest even), and
+// each 16-deep K sub-step runs al*bh + ah*bl + ah*bh on v_mfma_f32_32x32x16_bf16 with FP32
+// accumulation.  K step 32; LDS rows hold 32 bf16 of one row (+ 8 pad: 80-byte rows).
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t gg_f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ void gg_split(float x, uint16_t& hi, uint16_t& lo) {
+  hi = gg_f2bf(x);
+  lo = gg_f2bf(x - __uint_as_float((uint32_t)hi << 16));
+}
+
+template <int TM, int TN>
+__global__ __launch_bounds__(256) void k_grouped_bf16x3(GGDesc g) {
+  constexpr int BK = 32, LDK = 40;
+  constexpr int MB = TM / 64, NB = TN / 64;
+  constexpr int NA = TM * BK / 1024, NBL = TN * BK / 1024;
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][TM][LDK];  // [hi/lo][m][k]
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][TN][LDK];  // [hi/lo][n][k]
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
+  const float alpha = (float)g.alpha.re, beta = (float)g.beta.re;
+  for (int64_t tile = blockIdx.x; tile < g.total_tiles; tile += gridDim.x) {
+    const int p = g.map[tile];
+    const GGProb pr = g.probs[p];
+    const int64_t r = tile - g.first[p];
+    const int64_t per = (int64_t)pr.tiles_m * pr.tiles_n;
+    const int64_t batch = r / per;
+    const int64_t rr = r - batch * per;
+    const int64_t m0 = (rr % pr.tiles_m) * TM, n0 = (rr / pr.tiles_m) * TN;
+    const int64_t M = pr.m, N = pr.n, K = pr.k;
+    const float* __restrict__ A = (const float*)pr.a + batch * M * K;
+    const float* __restrict__ B = (const float*)pr.b + batch * K * N;
+    floatx16 acc[NB][MB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < MB; ++j)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.f;
+    float4 ra[NA], rb[NBL];
+    auto load = [&](int64_t k0) __attribute__((always_inline)) {
+#pragma unroll
+      for (int q = 0; q < NA; ++q) {
+        const int e = t + 256 * q;
+        const int64_t m = m0 + (e % (TM / 4)) * 4, k = k0 + e / (TM / 4);
+        const float* src = A + m + k * M;
+        if (pr.vec_a && m + 4 <= M && k < K) {
+          ra[q] = *(const float4*)src;
+        } else {
+          ra[q].x = (m < M && k < K) ? src[0] : 0.f;
+          ra[q].y = (m + 1 < M && k < K) ? src[1] : 0.f;
+          ra[q].z = (m + 2 < M && k < K) ? src[2] : 0.f;
+          ra[q].w = (m + 3 < M && k < K) ? src[3] : 0.f;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NBL; ++q) {
+        const int e = t + 256 * q;
+        const int64_t k = k0 + (e % (BK / 4)) * 4, n = n0 + e / (BK / 4);
+        const float* src = B + k + n * K;
+        if (pr.vec_b && k + 4 <= K && n < N) {
+          rb[q] = *(const float4*)src;
+        } else {
+          rb[q].x = (k < K && n < N) ? src[0] : 0.f;
+          rb[q].y = (k + 1 < K && n < N) ? src[1] : 0.f;
+          rb[q].z = (k + 2 < K && n < N) ? src[2] : 0.f;
+          rb[q].w = (k + 3 < K && n < N) ? src[3] : 0.f;
+        }
+      }
+    };
+    load(0);
+    for (int64_t k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+      for (int q = 0; q < NA; ++q) {
+        const int e = t + 256 * q;
+        const int m = (e % (TM / 4)) * 4, k = e / (TM / 4);
+        const float v[4] = {ra[q].x, ra[q].y, ra[q].z, ra[q].w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          uint16_t h, l;
+          gg_split(v[i], h, l);
+          sA[0][m + i][k] = h;
+          sA[1][m + i][k] = l;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NBL; ++q) {
+        const int e = t + 256 * q;
+        const int k = (e % (BK / 4)) * 4, n = e / (BK / 4);
+        uint16_t h[4], l[4];
+        gg_split(rb[q].x, h[0], l[0]);
+        gg_split(rb[q].y, h[1], l[1]);
+        gg_split(rb[q].z, h[2], l[2]);
+        gg_split(rb[q].w, h[3], l[3]);
+        uint2 ph, pl;
+        ph.x = h[0] | ((uint32_t)h[1] << 16); ph.y = h[2] | ((uint32_t)h[3] << 16);
+        pl.x = l[0] | ((uint32_t)l[1] << 16); pl.y = l[2] | ((uint32_t)l[3] << 16);
+        *(uint2*)&sB[0][n][k] = ph;
+        *(uint2*)&sB[1][n][k] = pl;
+      }
+      __syncthreads();
+      if (k0 + BK < K) load(k0 + BK);
+      if (m0 + wm < M && n0 + wn < N) {
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+          // lane l holds row l % 32, k = 16*s2 + 8*(l / 32) + 0..7
+          const int kk = 16 * s2 + 8 * (lane >> 5);
+          bf16x8_t ah[NB], al[NB], bh[MB], bl[MB];
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb) {
+            ah[nb] = *(const bf16x8_t*)&sB[0][wn + nb * 32 + (lane & 31)][kk];
+            al[nb] = *(const bf16x8_t*)&sB[1][wn + nb * 32 + (lane & 31)][kk];
+          }
+#pragma unroll
+          for (int mb = 0; mb < MB; ++mb) {
+            bh[mb] = *(const bf16x8_t*)&sA[0][wm + mb * 32 + (lane & 31)][kk];
+            bl[mb] = *(const bf16x8_t*)&sA[1][wm + mb * 32 + (lane & 31)][kk];
+          }
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) {
+              acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[nb], bh[mb], acc[nb][mb], 0, 0, 0);
+              acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[nb], bl[mb], acc[nb][mb], 0, 0, 0);
+              acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[nb], bh[mb], acc[nb][mb], 0, 0, 0);
+            }
+        }
+      }
+      __syncthreads();
+    }
+    float* __restrict__ C = (float*)pr.c + batch * M * N;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int64_t m = m0 + wm + mb * 32 + (lane & 31);
+          const int64_t n = n0 + wn + nb * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+          if (m < M && n < N) {
+            float* yp = C + m + n * M;
+            *yp = epilogue<OpsF32>(g.mode, alpha, beta, acc[nb][mb][q], yp);
+          }
+        }
+  }
+}
+
+// ---- grouped GEMM, one wave per 32 x 32 tile ----------------------------------------------
+// No LDS and no barriers: every wave owns a tile and loads its MFMA fragments straight from
+// global memory (A along m: 32 lanes read 128 contiguous bytes per k; B along k: 16 bytes per
+// lane), D K-steps ahead in registers, so many independent loads are in flight per SIMD.
+// Lane l holds fragment row l % 32 (n for the B tile, m for the A tile); lanes 0-31 take
+// k = k0 + s and lanes 32-63 k = k0 + 4 + s in the s-th FP32 MFMA of a K-step of 8.
+
+template <int D>
+__global__ __launch_bounds__(256) void k_grouped_f32_wave(GGDesc g) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  const float alpha = (float)g.alpha.re, beta = (float)g.beta.re;
+  for (int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < g.total_tiles; tile += nw) {
+    const int p = g.map[tile];
+    const GGProb pr = g.probs[p];
+    const int64_t r = tile - g.first[p];
+    const int64_t per = (int64_t)pr.tiles_m * pr.tiles_n;
+    const int64_t batch = r / per;
+    const int64_t rr = r - batch * per;
+    const int64_t m0 = (rr % pr.tiles_m) * 32, n0 = (rr / pr.tiles_m) * 32;
+    const int64_t M = pr.m, N = pr.n, K = pr.k;
+    const float* __restrict__ A = (const float*)pr.a + batch * M * K;
+    const float* __restrict__ B = (const float*)pr.b + batch * K * N;
+    const int64_t m = m0 + (lane & 31), n = n0 + (lane & 31);
+    const int kq = 4 * (lane >> 5);
+    const bool mok = m < M, nok = n < N;
+    const float* __restrict__ ap = A + (mok ? m : 0);
+    const float* __restrict__ bp = B + (nok ? n : 0) * K;
+    float4 fa[D], fb[D];
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    // rows outside the tile read row 0 (clamped pointers) and are zeroed after the load
+    auto load = [&](int d, int64_t k0) __attribute__((always_inline)) {
+      const int64_t k = k0 + kq;
+      if (k + 4 <= K) {
+        const float* bk = bp + k;
+        const float4 v = pr.vec_b ? *(const float4*)bk : make_float4(bk[0], bk[1], bk[2], bk[3]);
+        const float* ak = ap + k * M;
+        const float4 u = make_float4(ak[0], ak[M], ak[2 * M], ak[3 * M]);
+        fa[d] = nok ? v : z4;
+        fb[d] = mok ? u : z4;
+      } else {
+        fa[d].x = (nok && k < K) ? bp[k] : 0.f;
+        fa[d].y = (nok && k + 1 < K) ? bp[k + 1] : 0.f;
+        fa[d].z = (nok && k + 2 < K) ? bp[k + 2] : 0.f;
+        fa[d].w = (nok && k + 3 < K) ? bp[k + 3] : 0.f;
+        fb[d].x = (mok && k < K) ? ap[k * M] : 0.f;
+        fb[d].y = (mok && k + 1 < K) ? ap[(k + 1) * M] : 0.f;
+        fb[d].z = (mok && k + 2 < K) ? ap[(k + 2) * M] : 0.f;
+        fb[d].w = (mok && k + 3 < K) ? ap[(k + 3) * M] : 0.f;
+      }
+    };
+    floatx16 acc;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) load(d, 8 * d);
+    for (int64_t k0 = 0; k0 < K; k0 += 8 * D) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (k0 + 8 * d < K) {
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[d].x, fb[d].x, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[d].y, fb[d].y, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[d].z, fb[d].z, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[d].w, fb[d].w, acc, 0, 0, 0);
+          if (k0 + 8 * (d + D) < K) load(d, k0 + 8 * (d + D));
+        }
+      }
+    }
+    float* __restrict__ C = (float*)pr.c + batch * M * N;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int64_t nn = n0 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+      if (mok && nn < N) {
+        float* yp = C + m + nn * M;
+        *yp = epilogue<OpsF32>(g.mode, alpha, beta, acc[q], yp);
+      }
+    }
+  }
+}
+
+// BF16x3 form: K-steps of 16, lane l holds k = k0 + 8*(l/32) + 0..7 of its row, split into
+// bf16 hi/lo in registers; three v_mfma_f32_32x32x16_bf16 per step.
+typedef __bf16 bf16x8w_t __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8w_t gg_pack8(const uint16_t (&h)[8]) {
+  uint32_t u[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) u[i] = (uint32_t)h[2 * i] | ((uint32_t)h[2 * i + 1] << 16);
+  bf16x8w_t v;
+  __builtin_memcpy(&v, u, 16);
+  return v;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_grouped_bf16x3_wave(GGDesc g) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  const float alpha = (float)g.alpha.re, beta = (float)g.beta.re;
+  for (int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < g.total_tiles; tile += nw) {
+    const int p = g.map[tile];
+    const GGProb pr = g.probs[p];
+    const int64_t r = tile - g.first[p];
+    const int64_t per = (int64_t)pr.tiles_m * pr.tiles_n;
+    const int64_t batch = r / per;
+    const int64_t rr = r - batch * per;
+    const int64_t m0 = (rr % pr.tiles_m) * 32, n0 = (rr / pr.tiles_m) * 32;
+    const int64_t M = pr.m, N = pr.n, K = pr.k;
+    const float* __restrict__ A = (const float*)pr.a + batch * M * K;
+    const float* __restrict__ B = (const float*)pr.b + batch * K * N;
+    const int64_t m = m0 + (lane & 31), n = n0 + (lane & 31);
+    const int kq = 8 * (lane >> 5);
+    const bool mok = m < M, nok = n < N;
+    const float* __restrict__ ap = A + (mok ? m : 0);
+    const float* __restrict__ bp = B + (nok ? n : 0) * K;
+    float fa[D][8], fb[D][8];
+    auto load = [&](int d, int64_t k0) __attribute__((always_inline)) {
+      const int64_t k = k0 + kq;
+      if (k + 8 <= K) {
+        const float* bk = bp + k;
+        if (pr.vec_b) {
+          const float4 u = *(const float4*)bk, v = *(const float4*)(bk + 4);
+          fa[d][0] = u.x; fa[d][1] = u.y; fa[d][2] = u.z; fa[d][3] = u.w;
+          fa[d][4] = v.x; fa[d][5] = v.y; fa[d][6] = v.z; fa[d][7] = v.w;
+        } else {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) fa[d][i] = bk[i];
+        }
+        const float* ak = ap + k * M;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) fb[d][i] = ak[i * M];
+        if (!nok)
+#pragma unroll
+          for (int i = 0; i < 8; ++i) fa[d][i] = 0.f;
+        if (!mok)
+#pragma unroll
+          for (int i = 0; i < 8; ++i) fb[d][i] = 0.f;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) fa[d][i] = (nok && k + i < K) ? bp[k + i] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) fb[d][i] = (mok && k + i < K) ? ap[(k + i) * M] : 0.f;
+      }
+    };
+    floatx16 acc;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) load(d, 16 * d);
+    for (int64_t k0 = 0; k0 < K; k0 += 16 * D) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (k0 + 16 * d < K) {
+          uint16_t ah[8], al[8], bh[8], bl[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            gg_split(fa[d][i], ah[i], al[i]);
+            gg_split(fb[d][i], bh[i], bl[i]);
+          }
+          const bf16x8w_t vah = gg_pack8(ah), val = gg_pack8(al), vbh = gg_pack8(bh), vbl = gg_pack8(bl);
+          if (k0 + 16 * (d + D) < K) load(d, k0 + 16 * (d + D));
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(val, vbh, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(vah, vbl, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(vah, vbh, acc, 0, 0, 0);
+        }
+      }
+    }
+    float* __restrict__ C = (float*)pr.c + batch * M * N;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int64_t nn = n0 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+      if (mok && nn < N) {
+        float* yp = C + m + nn * M;
+        *yp = epilogue<OpsF32>(g.mode, alpha, beta, acc[q], yp);
+      }
+    }
+  }
+}
+
+// ---- grouped GEMM, semiring VALU tile (any element type) --------------------------------
+
+template <class O>
+__global__ __launch_bounds__(256) void k_grouped_valu(GGDesc g) {
+  using T = typename O::T;
+  constexpr int BM = 64, BN = 64, BK = 16;
+  __shared__ T As[BK][BM + 1];
+  __shared__ T Bs[BK][BN + 1];
+  const int t = threadIdx.x, tm = t & 15, tn = t >> 4;
+  const T alpha = O::from(g.alpha), beta = O::from(g.beta);
+  for (int64_t tile = blockIdx.x; tile < g.total_tiles; tile += gridDim.x) {
+    const int p = g.map[tile];
+    const GGProb pr = g.probs[p];
+    const int64_t r = tile - g.first[p];
+    const int64_t per = (int64_t)pr.tiles_m * pr.tiles_n;
+    const int64_t batch = r / per;
+    const int64_t rr = r - batch * per;
+    const int64_t m0 = (rr % pr.tiles_m) * BM, n0 = (rr / pr.tiles_m) * BN;
+    const int64_t M = pr.m, N = pr.n, K = pr.k;
+    const T* __restrict__ A = (const T*)pr.a + batch * M * K;
+    const T* __restrict__ B = (const T*)pr.b + batch * K * N;
+    T acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = O::zero();
+    for (int64_t k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = t + 256 * q;
+        const int mm = e & 63, kk = e >> 6;  // A: m fastest
+        const int64_t gm = m0 + mm, gk = k0 + kk;
+        As[kk][mm] = (gm < M && gk < K) ? A[gm + gk * M] : O::zero();
+        const int kb = e & 15, nn = e >> 4;  // B: k fastest
+        const int64_t gn = n0 + nn, gkb = k0 + kb;
+        Bs[kb][nn] = (gn < N && gkb < K) ? B[gkb + gn * K] : O::zero();
+      }
+      __syncthreads();
+      const int kn = (int)((K - k0) < BK ? (K - k0) : BK);
+      for (int kk = 0; kk < kn; ++kk) {
+        T a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = As[kk][tm + 16 * i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tn + 16 * j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = O::fma(a[i], b[j], acc[i][j]);
+      }
+      __syncthreads();
+    }
+    T* __restrict__ C = (T*)pr.c + batch * M * N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t m = m0 + tm + 16 * i, n = n0 + tn + 16 * j;
+        if (m < M && n < N) {
+          T* yp = C + m + n * M;
+          *yp = epilogue<O>(g.mode, alpha, beta, acc[i][j], yp);
+        }
+      }
+  }
+}
+
+void gg_tile_shape(int dt, int variant, int& tm, int& tn) {
+  if (dt != F32) { tm = tn = 64; return; }
+  if (variant == GG_F32_W32 || variant == GG_BF16X3_W32) { tm = tn = 32; return; }
+  if (variant == GG_F32_128 || variant == GG_BF16X3_128) { tm = tn = 128; return; }
+  tm = tn = 64;
+}
+
+void launch_grouped_gemm(int dt, int variant, const GGDesc& g, int grid, hipStream_t st) {
+  if (g.total_tiles <= 0) return;
+  OME_DRY_RETURN(1);
+  TimedScope ts(KC_GROUPED, st);
+  if (dt == F32) {
+    if (variant == GG_F32_W32)
+      hipLaunchKernelGGL((k_grouped_f32_wave<4>), dim3(grid), dim3(256), 0, st, g);
+    else if (variant == GG_BF16X3_W32)
+      hipLaunchKernelGGL((k_grouped_bf16x3_wave<2>), dim3(grid), dim3(256), 0, st, g);
+    else if (variant == GG_BF16X3_64)
+      hipLaunchKernelGGL((k_grouped_bf16x3<64, 64>), dim3(grid), dim3(256), 0, st, g);
+    else if (variant == GG_BF16X3_128)
+      hipLaunchKernelGGL((k_grouped_bf16x3<128, 128>), dim3(grid), dim3(256), 0, st, g);
+    else if (variant == GG_F32_128)
+      hipLaunchKernelGGL((k_grouped_f32<128, 128, 16>), dim3(grid), dim3(256), 0, st, g);
+    else if (variant == GG_F32_64_BK32)
+      hipLaunchKernelGGL((k_grouped_f32<64, 64, 32>), dim3(grid), dim3(256), 0, st, g);
+    else
+      hipLaunchKernelGGL((k_grouped_f32<64, 64, 16>), dim3(grid), dim3(256), 0, st, g);
+    return;
+  }
+  OME_DISPATCH(dt, O, {
+    GGDesc gg = g;
+    gg.mode = epi_mode<O>(g.alpha, g.beta);
+    hipLaunchKernelGGL(k_grouped_valu<O>, dim3(grid), dim3(256), 0, st, gg);
+  });
+}
+
+// ---- grouped permute ----------------------------------------------------------------------
+
+template <class O>
+__global__ __launch_bounds__(256) void k_grouped_permute(GPDesc g) {
+  using T = typename O::T;
+  __shared__ T tile[4096 + 256];  // tla rows of tlb + 1 (tla <= 256)
+  __shared__ int64_t soff_a[256], soff_b[64];
+  const T sx = O::from(g.sx), sy = O::from(g.sy);
+  for (int64_t blk = blockIdx.x; blk < g.total_blocks; blk += gridDim.x) {
+    const int p = g.map[blk];
+    const GPProb& pr = g.probs[p];
+    const T* __restrict__ x = (const T*)pr.x;
+    T* __restrict__ y = (T*)pr.y;
+    int64_t b = blk - g.first[p];
+    if (pr.tiled) {
+      const int64_t tbk = b % pr.tb;  // b-tiles fastest: consecutive blocks read neighbouring x
+      b /= pr.tb;
+      const int64_t tak = b % pr.ta;
+      uint32_t rest = (uint32_t)(b / pr.ta);
+      int64_t ibase = 0, obase = 0;
+      for (int q = 0; q < pr.nr; ++q) {
+        const uint32_t qq = fdiv_q(rest, pr.rdv[q]);
+        const uint32_t v = rest - qq * pr.rdv[q].d;
+        rest = qq;
+        ibase += (int64_t)v * pr.r_is[q];
+        obase += (int64_t)v * pr.r_os[q];
+      }
+      const int tla = pr.tla, tlb = pr.tlb, ld = tlb + 1;
+      const uint32_t cnt = (uint32_t)(tla * tlb);  // <= 4096: at most 16 per thread
+      // the input offset of element (a, b) is b + fa(a) and its output offset a + fb(b):
+      // tabulate fa over the tile's a values and fb over its b values once
+      for (int u = threadIdx.x; u < tla; u += 256) {
+        uint32_t ra = (uint32_t)(tak * tla + u);
+        int64_t off = 0;
+        for (int q = 0; q < pr.no; ++q) {
+          const uint32_t qq = fdiv_q(ra, pr.odv[q]);
+          off += (int64_t)(ra - qq * pr.odv[q].d) * pr.o_is[q];
+          ra = qq;
+        }
+        soff_a[u] = off;
+      }
+      for (int u = threadIdx.x; u < tlb; u += 256) {
+        uint32_t rb = (uint32_t)(tbk * tlb + u);
+        int64_t off = 0;
+        for (int q = 0; q < pr.ni; ++q) {
+          const uint32_t qq = fdiv_q(rb, pr.idv[q]);
+          off += (int64_t)(rb - qq * pr.idv[q].d) * pr.i_os[q];
+          rb = qq;
+        }
+        soff_b[u] = off;
+      }
+      __syncthreads();
+      const int64_t na = pr.pa - tak * tla, nb = pr.pb - tbk * tlb;  // valid extents
+      const T* __restrict__ xb = x + ibase + tbk * tlb;
+      T* __restrict__ yb = y + obase + tak * tla;
+      // read: b (contiguous in x) fastest across lanes; the loads of a batch are all issued
+      // before any LDS store (one batch of 16, or two of 8 for 16-byte elements)
+      constexpr int NBT = sizeof(T) >= 16 ? 8 : 16;
+#pragma unroll
+      for (int h = 0; h < 16; h += NBT) {
+        T v[NBT];
+#pragma unroll
+        for (int j = 0; j < NBT; ++j) {
+          const uint32_t e = threadIdx.x + 256u * (h + j);
+          const uint32_t al = fdiv_q(e, pr.tlb_div), bl = e - al * (uint32_t)tlb;
+          if (e < cnt && al < na && bl < nb) v[j] = xb[bl + soff_a[al]];
+        }
+#pragma unroll
+        for (int j = 0; j < NBT; ++j) {
+          const uint32_t e = threadIdx.x + 256u * (h + j);
+          const uint32_t al = fdiv_q(e, pr.tlb_div), bl = e - al * (uint32_t)tlb;
+          if (e < cnt) tile[al * ld + bl] = v[j];
+        }
+      }
+      __syncthreads();
+      // write: a (contiguous in y) fastest across lanes
+#pragma unroll 4
+      for (int j = 0; j < 16; ++j) {
+        const uint32_t e = threadIdx.x + 256u * j;
+        if (e >= cnt) break;
+        const uint32_t bl = fdiv_q(e, pr.tla_div), al = e - bl * (uint32_t)tla;
+        if (al < na && bl < nb) {
+          T* yp = yb + al + soff_b[bl];
+          *yp = epilogue<O>(g.mode, sx, sy, tile[al * ld + bl], yp);
+        }
+      }
+      __syncthreads();
+    } else {
+      // 1024 consecutive outputs along the flattened output
+      const int64_t base = b * 1024;
+      T v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t o = base + threadIdx.x + 256 * j;
+        if (o < pr.n) {
+          uint32_t rem = (uint32_t)o;
+          int64_t off = 0;
+          for (int q = 0; q < pr.rank; ++q) {
+            const uint32_t qq = fdiv_q(rem, pr.odiv[q]);
+            off += (int64_t)(rem - qq * pr.odiv[q].d) * pr.istride[q];
+            rem = qq;
+          }
+          v[j] = x[off];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t o = base + threadIdx.x + 256 * j;
+        if (o < pr.n) y[o] = epilogue<O>(g.mode, sx, sy, v[j], y + o);
+      }
+    }
+  }
+}
+
+void launch_grouped_permute(int dt, const GPDesc& g, int grid, hipStream_t st) {
+  if (g.total_blocks <= 0) return;
+  OME_DRY_RETURN(1);
+  TimedScope ts(KC_PERMUTE, st);
+  OME_DISPATCH(dt, O, {
+    GPDesc gg = g;
+    gg.mode = epi_mode<O>(g.sx, g.sy);
+    hipLaunchKernelGGL(k_grouped_permute<O>, dim3(grid), dim3(256), 0, st, gg);
+  });
+}
+
+}  // namespace ome

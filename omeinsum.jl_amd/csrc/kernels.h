@@ -75,6 +75,68 @@ void launch_prep_b(int dt, bool rdouble, int algo, const GemmDesc& d, void* out,
 void launch_combine(int dt, bool rdouble, int algo, const GemmDesc& d, const void* c,
                     hipStream_t st);
 
+// Grouped launches (one persistent launch over many problems).
+struct GGProb {  // one problem of a grouped GEMM: dense column-major, batch-major
+  const void* a;
+  const void* b;
+  void* c;
+  int64_t m, n, k;
+  int32_t tiles_m, tiles_n;
+  int32_t vec_a, vec_b;  // 16-byte vector loads allowed along m (A) / k (B)
+};
+struct GGDesc {
+  const GGProb* probs;   // device
+  const int64_t* first;  // device: first tile of each problem (ascending)
+  const int32_t* map;    // device: problem of each tile
+  int nprob;
+  int64_t total_tiles;
+  Scal alpha, beta;
+  int mode;
+};
+enum GGVariant : int {
+  GG_F32_64 = 0, GG_F32_64_BK32 = 1, GG_F32_128 = 2, GG_BF16X3_64 = 3, GG_BF16X3_128 = 4,
+  GG_F32_W32 = 5, GG_BF16X3_W32 = 6, GG_NVARIANTS = 7
+};
+void gg_tile_shape(int dt, int variant, int& tm, int& tn);
+void launch_grouped_gemm(int dt, int variant, const GGDesc& g, int grid, hipStream_t st);
+
+constexpr int GP_MAXR = 8;
+// One permute.  Plain path (tiled == 0): output dims (fastest first) with their input strides.
+// Tiled path: the leading output dims (group O, linear index a, contiguous in y) and the
+// leading input dims (group I, linear index b, contiguous in x) are disjoint; a block moves
+// a tla x tlb (a, b) tile through LDS for one value of the remaining dims (group R), so both
+// the reads (along b) and the writes (along a) are coalesced.
+struct GPProb {
+  const void* x;
+  void* y;
+  int64_t n;
+  int32_t rank;
+  int32_t tiled;
+  FDiv odiv[GP_MAXR];        // plain: output extents
+  int64_t istride[GP_MAXR];  // plain: input stride per output dim
+  int32_t no, ni, nr;        // tiled: dims in O, I, R
+  int64_t pa, pb;            // tiled: extent of a (product over O) and of b (over I)
+  int64_t ta, tb;            // tiled: tiles along a and b
+  int32_t tla, tlb;          // tiled: tile extent along a and b (tla * tlb <= 4096)
+  FDiv tlb_div, tla_div;
+  FDiv odv[4];               // O dims (output order), their input strides
+  int64_t o_is[4];
+  FDiv idv[4];               // I dims (input order), their output strides
+  int64_t i_os[4];
+  FDiv rdv[GP_MAXR];         // R dims, input and output strides
+  int64_t r_is[GP_MAXR], r_os[GP_MAXR];
+};
+struct GPDesc {
+  const GPProb* probs;
+  const int64_t* first;  // first block of each problem
+  const int32_t* map;    // problem of each block
+  int nprob;
+  int64_t total_blocks;
+  Scal sx, sy;
+  int mode;
+};
+void launch_grouped_permute(int dt, const GPDesc& g, int grid, hipStream_t st);
+
 int mfma_plan_ksplit(const GemmDesc& d, int requested);
 int64_t mfma_workspace(int dt, const GemmDesc& d, int ksplit);
 void launch_gemm_mfma(int prec, GemmDesc d, int ksplit, hipStream_t st);

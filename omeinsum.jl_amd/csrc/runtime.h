@@ -42,6 +42,7 @@ struct Options {
   int use_graphs = 1;
   int autotune = 1;
   int gemm_kernel = GEMM_AUTO;
+  int grouped_variant = -1;  // -1: autotuned; else the grouped GEMM tile variant
 };
 
 struct Status {
@@ -80,8 +81,15 @@ struct Bump {
 };
 }  // namespace ome
 
+namespace ome {
+struct GroupState;  // grouped-launch descriptor cache and autotuner (grouped.hip)
+void destroy_group_state(ome_ctx* ctx);
+}  // namespace ome
+
 struct ome_ctx {
   ome::Bump* bump = nullptr;
+  ome::GroupState* gstate = nullptr;
+  int last_grouped_choice = -1;  // 2 * variant + persistent of the last grouped GEMM
   int device = 0;
   hipStream_t stream = nullptr;
   std::recursive_mutex mu;

@@ -2,14 +2,6 @@
 #include "runtime.h"
 
 extern "C" {
-int ome_grouped_gemm(ome_ctx* ctx, int, int, const int64_t*, const void* const*,
-                     const void* const*, void* const*, const ome_scalar*, const ome_scalar*) {
-  if (ctx) ctx->last_error = "grouped GEMM not available in this build";
-  return OME_ERR_UNSUPPORTED;
-}
-}
-
-extern "C" {
 static int unsup(const char* what) { (void)what; return OME_ERR_UNSUPPORTED; }
 int ome_server_create(ome_ctx*, size_t, int, ome_server**) { return unsup("server"); }
 int ome_server_submit(ome_server*, ome_tree*, const void* const*, void*, int64_t*) { return unsup("server"); }

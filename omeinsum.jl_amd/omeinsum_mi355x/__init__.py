@@ -16,7 +16,8 @@ from .contractionorder import (GreedyMethod, TreeSASlicer, optimize_code, slice_
 _LAZY = {
     "einsum": "ein_ops", "einsum_inplace": "ein_ops", "loop_einsum": "ein_ops",
     "loop_einsum_inplace": "ein_ops", "allow_loops": "ein_ops", "tensorpermute": "ein_ops",
-    "batched_gemm": "ein_ops", "grouped_gemm": "ein_ops", "asarray": "device",
+    "batched_gemm": "ein_ops", "grouped_gemm": "ein_ops", "grouped_permute": "ein_ops",
+    "GroupedGemm": "ein_ops", "GroupedPermute": "ein_ops", "asarray": "device",
     "asscalar": "device", "Tropical": "device", "to_device": "device", "to_numpy": "device",
     "EinsumBackend": "backends", "DefaultBackend": "backends", "MI355XBackend": "backends",
     "set_einsum_backend": "backends", "get_einsum_backend": "backends",

@@ -173,30 +173,109 @@ def batched_gemm(ta, tb, alpha, A, B, beta, C):
     return C
 
 
-def grouped_gemm(problems, alpha=True, beta=False):
-    """Many independent batched GEMMs of uneven shapes in one persistent launch.
+class GroupedGemm:
+    """Many independent batched GEMMs of uneven shapes in one launch (ome_grouped_gemm).
     problems: list of (A, B, C) row-major tensors with A (batch, m, k), B (batch, k, n),
-    C (batch, m, n) (torch convention).  C = alpha*A@B + beta*C for each problem."""
-    if not problems:
-        return
-    ts, trop, T = _unwrap([p[0] for p in problems] + [p[1] for p in problems])
-    na = len(problems)
-    ct0 = problems[0][2]
-    ctx = _lib.context(ct0.device.index)
-    code = dtype_code(ct0.dtype, trop)
-    dims = (ctypes.c_int64 * (4 * na))()
-    pa = (ctypes.c_void_p * na)()
-    pb = (ctypes.c_void_p * na)()
-    pc = (ctypes.c_void_p * na)()
-    for g, (A, B, C) in enumerate(problems):
-        a, b = ts[g], ts[na + g]
-        batch, m, k = a.shape
-        n = b.shape[2]
-        # column-major view: C^T (n, m, batch) = B^T (n, k) * A^T (k, m)
-        dims[4 * g:4 * g + 4] = [n, m, k, batch]
-        pa[g] = b.data_ptr()
-        pb[g] = a.data_ptr()
-        pc[g] = C.data_ptr()
-    al, be = scalar(alpha, code), scalar(beta, code)
-    _lib.check(ctx.lib.ome_grouped_gemm(ctx.h, code, na, dims, pa, pb, pc, ctypes.byref(al),
-                                        ctypes.byref(be)), ctx.h)
+    C (batch, m, n) (torch convention); calling the object computes C = alpha*A@B + beta*C
+    for every problem.  The argument arrays are built once, so a repeated call costs one
+    library call (the library caches the device descriptors of the same problem list)."""
+
+    def __init__(self, problems, alpha=True, beta=False):
+        self.n = len(problems)
+        if not self.n:
+            return
+        ts, trop, T = _unwrap([p[0] for p in problems] + [p[1] for p in problems])
+        na = self.n
+        ct0 = problems[0][2]
+        ct0 = ct0.data if isinstance(ct0, Tropical) else ct0
+        self.ctx = _lib.context(ct0.device.index)
+        self.code = dtype_code(T, trop)
+        self.dims = (ctypes.c_int64 * (4 * na))()
+        self.pa = (ctypes.c_void_p * na)()
+        self.pb = (ctypes.c_void_p * na)()
+        self.pc = (ctypes.c_void_p * na)()
+        self.keep = []
+        for g, (A, B, C) in enumerate(problems):
+            a, b = ts[g], ts[na + g]
+            ct = C.data if isinstance(C, Tropical) else C
+            batch, m, k = a.shape
+            n = b.shape[2]
+            if tuple(b.shape) != (batch, k, n) or tuple(ct.shape) != (batch, m, n):
+                raise ValueError("problem %d: shapes %s x %s -> %s do not match" %
+                                 (g, tuple(a.shape), tuple(b.shape), tuple(ct.shape)))
+            if not ct.is_contiguous() or ct.dtype != T:
+                raise ValueError("problem %d: C must be a contiguous %s tensor" % (g, T))
+            self.keep.append((a, b, ct))
+            # column-major view: C^T (n, m, batch) = B^T (n, k) * A^T (k, m)
+            self.dims[4 * g:4 * g + 4] = [n, m, k, batch]
+            self.pa[g] = b.data_ptr()
+            self.pb[g] = a.data_ptr()
+            self.pc[g] = ct.data_ptr()
+        self.al, self.be = scalar(alpha, self.code), scalar(beta, self.code)
+
+    def __call__(self):
+        if not self.n:
+            return
+        self.ctx.sync_stream()
+        _lib.check(self.ctx.lib.ome_grouped_gemm(self.ctx.h, self.code, self.n, self.dims, self.pa,
+                                                 self.pb, self.pc, ctypes.byref(self.al),
+                                                 ctypes.byref(self.be)), self.ctx.h)
+
+
+def grouped_gemm(problems, alpha=True, beta=False):
+    """One call of GroupedGemm(problems, alpha, beta)."""
+    GroupedGemm(problems, alpha, beta)()
+
+
+class GroupedPermute:
+    """Many tensorpermute! calls (src/utils.jl:122-156) in one launch (ome_grouped_permute):
+    calling the object sets ys[g] = sx * xs[g].permute(perms[g]) + sy * ys[g] (torch's axis
+    convention: output axis p is input axis perms[g][p])."""
+
+    def __init__(self, xs, perms, ys, sx=True, sy=False):
+        self.n = len(xs)
+        if not self.n:
+            return
+        ts, trop, T = _unwrap(list(xs))
+        self.ctx = _lib.context(ts[0].device.index)
+        self.code = dtype_code(T, trop)
+        n = self.n
+        self.nd = (ctypes.c_int32 * n)()
+        dims, cperm = [], []
+        self.pa = (ctypes.c_void_p * n)()
+        self.pc = (ctypes.c_void_p * n)()
+        self.keep = []
+        for g, (a, perm, y) in enumerate(zip(ts, perms, ys)):
+            yt = y.data if isinstance(y, Tropical) else y
+            r = a.dim()
+            if sorted(perm) != list(range(r)):
+                raise ValueError("problem %d: %s is not a permutation of %d axes" % (g, perm, r))
+            if tuple(yt.shape) != tuple(a.shape[p] for p in perm) or not yt.is_contiguous() \
+                    or yt.dtype != T:
+                raise ValueError("problem %d: output must be a contiguous %s tensor of the "
+                                 "permuted shape" % (g, T))
+            self.keep.append((a, yt))
+            self.nd[g] = r
+            # row-major axis p is column-major dim r-1-p
+            dims += [a.shape[r - 1 - q] for q in range(r)]
+            cperm += [r - 1 - perm[r - 1 - q] for q in range(r)]
+            self.pa[g] = a.data_ptr()
+            self.pc[g] = yt.data_ptr()
+        self.dims = (ctypes.c_int64 * max(len(dims), 1))(*dims)
+        self.perm = (ctypes.c_int32 * max(len(cperm), 1))(*cperm)
+        self.sx, self.sy = scalar(sx, self.code), scalar(sy, self.code)
+
+    def __call__(self):
+        if not self.n:
+            return
+        self.ctx.sync_stream()
+        _lib.check(self.ctx.lib.ome_grouped_permute(self.ctx.h, self.code, self.n, self.nd,
+                                                    self.dims, self.perm, self.pa, self.pc,
+                                                    ctypes.byref(self.sx), ctypes.byref(self.sy)),
+                   self.ctx.h)
+
+
+def grouped_permute(xs, perms, ys, sx=True, sy=False):
+    """One call of GroupedPermute(xs, perms, ys, sx, sy); returns ys."""
+    GroupedPermute(xs, perms, ys, sx, sy)()
+    return ys
