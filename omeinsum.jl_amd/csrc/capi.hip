@@ -273,6 +273,7 @@ int ome_set_option(ome_ctx* ctx, int key, int64_t v) {
       if (v < -1 || v > 2) return fail(ctx, OME_ERR_INVALID_ARG, "gemm kernel must be -1..2");
       ctx->opt.gemm_kernel = (int)v; break;
     case OME_OPT_KERNEL_TIMING: timing_enable(v != 0); break;
+    case OME_OPT_TREE_DEFER_MEMORY: ctx->opt.defer_tree_mem = (int)(v != 0); break;
     case OME_OPT_GROUPED_VARIANT:
       if (v < -1 || v >= GG_NVARIANTS) return fail(ctx, OME_ERR_INVALID_ARG, "grouped variant must be -1..6");
       ctx->opt.grouped_variant = (int)v; break;
@@ -295,6 +296,7 @@ int ome_get_option(ome_ctx* ctx, int key, int64_t* v) {
     case OME_OPT_KERNEL_TIMING: *v = timing_enabled() ? 1 : 0; break;
     case OME_OPT_GROUPED_VARIANT: *v = ctx->opt.grouped_variant; break;
     case OME_OPT_LAST_GROUPED_CHOICE: *v = ctx->last_grouped_choice; break;
+    case OME_OPT_TREE_DEFER_MEMORY: *v = ctx->opt.defer_tree_mem; break;
     default: return fail(ctx, OME_ERR_INVALID_ARG, "unknown option");
   }
   return OME_OK;

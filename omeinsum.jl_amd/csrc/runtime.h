@@ -43,6 +43,7 @@ struct Options {
   int autotune = 1;
   int gemm_kernel = GEMM_AUTO;
   int grouped_variant = -1;  // -1: autotuned; else the grouped GEMM tile variant
+  int defer_tree_mem = 0;    // 1: tree plans get device memory on first use, not at creation
 };
 
 struct Status {
@@ -152,5 +153,10 @@ Status run_gemm(ome_ctx* ctx, int dt, GemmDesc d, hipStream_t st, int precision)
 
 Scal scal_one(int dt);
 Scal scal_zero(int dt);
+
+// Request-loop access to tree plans (tree.hip).
+int64_t tree_mem_bytes(const ome_tree* t);
+Status tree_run_lent(ome_tree* t, const void* const* xs, void* y, char* mem, hipStream_t st);
+void tree_return_lent(ome_tree* t);
 
 }  // namespace ome

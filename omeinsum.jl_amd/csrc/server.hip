@@ -1,0 +1,282 @@
+// Continuous batching of tensor-network contraction requests (ome_server_*).
+//
+// A request is a planned tree (ome_tree) with its input and output buffers.  One host
+// scheduler thread per server runs the loop
+//   1. retire: a slot whose completion event has fired returns its request's device memory
+//      to the pool, marks the request done (completion time = when the event was seen) and
+//      frees the slot;
+//   2. admit: queued requests are scanned in arrival order and every one whose plan memory
+//      (arena + scratch + staging + accumulators, known exactly at plan time) fits the
+//      remaining HBM budget is started on a free slot: memory from the pool is lent to the
+//      plan, the slot's stream waits for the submitter's stream (the inputs are ready), the
+//      whole evaluation is enqueued, and a completion event is recorded;
+//   3. sleep briefly while requests run, or block until the next submission when idle.
+// Each slot has its own stream, so the steps of admitted requests overlap on the device, and
+// requests that arrive while others run are admitted as soon as a slot and memory are free.
+// Sliced requests (SlicedEinsum plans) keep their FP64 cross-slice accumulator in the lent
+// memory.  A tree must not be used elsewhere while its request is queued or running.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+
+#include "runtime.h"
+
+using namespace ome;
+
+namespace {
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Req {
+  int64_t ticket = 0;
+  ome_tree* tree = nullptr;
+  std::vector<const void*> xs;
+  void* y = nullptr;
+  hipEvent_t ready = nullptr;  // recorded on the submitter's stream
+  double t_submit = 0, t_start = 0, t_done = 0;
+  int status = 0;  // 0 queued, 1 running, 2 done, < 0 failed (negated ome_status)
+  std::string msg;
+  size_t bytes = 0;
+  char* mem = nullptr;
+};
+
+struct Slot {
+  hipStream_t st = nullptr;
+  hipEvent_t done = nullptr;
+  Req* req = nullptr;
+};
+
+}  // namespace
+
+struct ome_server {
+  ome_ctx* ctx = nullptr;
+  size_t budget = 0, in_use = 0;
+  std::vector<Slot> slots;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Req*> queue;
+  std::unordered_map<int64_t, Req*> reqs;
+  int64_t next = 1;
+  int active = 0;
+  bool stop = false;
+  std::thread th;
+};
+
+namespace {
+
+void retire(ome_server* s, Slot& sl, int status, const std::string& msg) {
+  Req* r = sl.req;
+  tree_return_lent(r->tree);
+  if (r->mem) s->ctx->pool->free(r->mem);
+  r->mem = nullptr;
+  s->in_use -= r->bytes;
+  r->t_done = now_ms();
+  r->status = status;
+  r->msg = msg;
+  sl.req = nullptr;
+  --s->active;
+}
+
+void scheduler(ome_server* s) {
+  hipSetDevice(s->ctx->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    bool progress = false;
+    // 1. retire finished requests
+    for (auto& sl : s->slots) {
+      if (!sl.req || sl.req->status != 1) continue;
+      const hipError_t q = hipEventQuery(sl.done);
+      if (q == hipErrorNotReady) continue;
+      if (q == hipSuccess) {
+        retire(s, sl, 2, "");
+      } else {
+        hipGetLastError();
+        retire(s, sl, -OME_ERR_DEVICE, std::string("request failed on the device: ") + hipGetErrorString(q));
+      }
+      progress = true;
+    }
+    if (progress) s->cv_done.notify_all();
+    // 2. admit queued requests that fit, in arrival order
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
+      Slot* free_slot = nullptr;
+      for (auto& sl : s->slots)
+        if (!sl.req) { free_slot = &sl; break; }
+      if (!free_slot) break;
+      Req* r = *it;
+      if (r->bytes > s->budget - s->in_use) { ++it; continue; }
+      it = s->queue.erase(it);
+      char* mem = r->bytes ? (char*)s->ctx->pool->alloc(r->bytes) : nullptr;
+      if (r->bytes && !mem) {
+        // the pool cannot serve the budget it was promised: fail this request
+        r->status = -OME_ERR_OOM;
+        r->msg = "device pool exhausted for an admitted request";
+        r->t_done = now_ms();
+        s->cv_done.notify_all();
+        continue;
+      }
+      r->mem = mem;
+      r->status = 1;
+      r->t_start = now_ms();
+      s->in_use += r->bytes;
+      ++s->active;
+      free_slot->req = r;
+      Slot& sl = *free_slot;
+      // enqueue outside the server lock (submitters keep going); the slot is claimed
+      lk.unlock();
+      hipStreamWaitEvent(sl.st, r->ready, 0);
+      Status st = tree_run_lent(r->tree, r->xs.data(), r->y, mem, sl.st);
+      hipError_t e = hipEventRecord(sl.done, sl.st);
+      lk.lock();
+      if (!st.good() || e != hipSuccess) {
+        if (e != hipSuccess) hipGetLastError();
+        hipStreamSynchronize(sl.st);
+        retire(s, sl, -(st.good() ? OME_ERR_DEVICE : st.code), st.good() ? "event record failed" : st.msg);
+        s->cv_done.notify_all();
+      }
+      progress = true;
+      it = s->queue.begin();  // the queue may have changed while unlocked
+    }
+    if (s->stop && s->queue.empty() && s->active == 0) break;
+    if (!progress) {
+      if (s->active == 0) {
+        s->cv_work.wait(lk, [s] { return s->stop || !s->queue.empty(); });
+      } else {
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        lk.lock();
+      }
+    }
+  }
+}
+
+int sfail(ome_server* s, int code, const std::string& msg) {
+  if (s && s->ctx) s->ctx->last_error = msg;
+  return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ome_server_create(ome_ctx* ctx, size_t hbm_budget, int max_active, ome_server** out) {
+  if (!ctx || !out || max_active < 1 || hbm_budget == 0) {
+    if (ctx) ctx->last_error = "invalid request-loop arguments";
+    return OME_ERR_INVALID_ARG;
+  }
+  hipSetDevice(ctx->device);
+  ome_server* s = new ome_server();
+  s->ctx = ctx;
+  s->budget = hbm_budget;
+  s->slots.resize(max_active);
+  for (auto& sl : s->slots) {
+    if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+      hipGetLastError();
+      for (auto& q : s->slots) {
+        if (q.st) hipStreamDestroy(q.st);
+        if (q.done) hipEventDestroy(q.done);
+      }
+      delete s;
+      ctx->last_error = "could not create the request-loop streams";
+      return OME_ERR_DEVICE;
+    }
+  }
+  s->th = std::thread(scheduler, s);
+  *out = s;
+  return OME_OK;
+}
+
+int ome_server_submit(ome_server* s, ome_tree* tree, const void* const* xs, void* y, int64_t* ticket) {
+  if (!s || !tree || !ticket) return sfail(s, OME_ERR_INVALID_ARG, "null argument");
+  ome_tree_info info{};
+  ome_get_tree_info(tree, &info);
+  Req* r = new Req();
+  r->tree = tree;
+  r->xs.assign(xs, xs + info.n_leaves);
+  r->y = y;
+  r->bytes = (size_t)tree_mem_bytes(tree);
+  if (r->bytes > s->budget) {
+    delete r;
+    return sfail(s, OME_ERR_OOM, "request needs " + std::to_string(tree_mem_bytes(tree)) +
+                                     " bytes of device memory, more than the request-loop budget");
+  }
+  hipSetDevice(s->ctx->device);
+  hipStream_t submit_stream;
+  {
+    std::lock_guard<std::recursive_mutex> g(s->ctx->mu);
+    submit_stream = s->ctx->stream;
+  }
+  if (hipEventCreateWithFlags(&r->ready, hipEventDisableTiming) != hipSuccess ||
+      hipEventRecord(r->ready, submit_stream) != hipSuccess) {
+    hipGetLastError();
+    if (r->ready) hipEventDestroy(r->ready);
+    delete r;
+    return sfail(s, OME_ERR_DEVICE, "could not record the submission event");
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->ticket = s->next++;
+    r->t_submit = now_ms();
+    s->reqs[r->ticket] = r;
+    s->queue.push_back(r);
+  }
+  s->cv_work.notify_one();
+  *ticket = r->ticket;
+  return OME_OK;
+}
+
+int ome_server_poll(ome_server* s, int64_t ticket, int* status) {
+  if (!s || !status) return sfail(s, OME_ERR_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return sfail(s, OME_ERR_INVALID_ARG, "unknown ticket");
+  *status = it->second->status;
+  return OME_OK;
+}
+
+int ome_server_wait(ome_server* s, int64_t ticket, double* latency_ms) {
+  if (!s) return OME_ERR_INVALID_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return sfail(s, OME_ERR_INVALID_ARG, "unknown ticket");
+  Req* r = it->second;
+  s->cv_done.wait(lk, [r] { return r->status >= 2 || r->status < 0; });
+  if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+  const int status = r->status;
+  const std::string msg = r->msg;
+  s->reqs.erase(it);
+  lk.unlock();
+  hipEventDestroy(r->ready);
+  delete r;
+  if (status < 0) return sfail(s, -status, msg);
+  return OME_OK;
+}
+
+int ome_server_destroy(ome_server* s) {
+  if (!s) return OME_ERR_INVALID_ARG;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  s->th.join();
+  hipSetDevice(s->ctx->device);
+  for (auto& sl : s->slots) {
+    hipStreamSynchronize(sl.st);
+    hipStreamDestroy(sl.st);
+    hipEventDestroy(sl.done);
+  }
+  for (auto& kv : s->reqs) {
+    hipEventDestroy(kv.second->ready);
+    delete kv.second;
+  }
+  delete s;
+  return OME_OK;
+}
+
+}  // extern "C"

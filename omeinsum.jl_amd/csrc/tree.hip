@@ -124,6 +124,8 @@ struct ome_tree {
   std::list<GraphEntry> graphs;  // most recent first
   hipStream_t cap_stream = nullptr;
   ome::Options opt;  // planning options frozen at creation
+  int64_t mem_bytes = 0;  // size of mem
+  bool owns_mem = true;   // mem came from the pool for this tree (else a request loop lent it)
   // gradient plan (lazily built)
   char* gmem = nullptr;
   int64_t g_arena = 0, g_scratch = 0;
@@ -308,6 +310,20 @@ static void accumulate_slice(ome_tree* t, int64_t sidx, void* acc, hipStream_t s
   launch_acc_strided(t->dt, t->acc_f64, d, st);
 }
 
+// Give the plan its own device memory if it has none (plans created with deferred memory,
+// or whose memory a request loop took back).  Recorded graphs hold the old addresses.
+static Status ensure_mem(ome_tree* t) {
+  if (t->mem) return Status::ok();
+  t->mem = (char*)t->ctx->pool->alloc((size_t)t->mem_bytes);
+  if (!t->mem)
+    return Status::err(OME_ERR_OOM, "device pool exhausted (tree arena of " +
+                                        std::to_string(t->mem_bytes) + " bytes)");
+  t->owns_mem = true;
+  for (auto& ge : t->graphs) hipGraphExecDestroy(ge.exec);
+  t->graphs.clear();
+  return Status::ok();
+}
+
 // One full evaluation (all slices) enqueued on st.
 static Status enqueue_eval(ome_tree* t, const std::vector<const void*>& xs, void* y,
                            const Scal& sx, const Scal& sy, hipStream_t st) {
@@ -469,11 +485,14 @@ int ome_tree_create(ome_ctx* ctx, int dtype, const int32_t* desc, int64_t desc_l
   for (auto& o : t->stage_off)
     if (o >= 0) o += stage_base;
   const int64_t total = t->arena_bytes + t->scratch_bytes + t->stage_bytes + t->acc_bytes + t->norm_bytes;
-  t->mem = (char*)ctx->pool->alloc((size_t)total);
-  if (!t->mem) {
-    delete t;
-    return tree_fail(ctx, Status::err(OME_ERR_OOM, "device pool exhausted (tree arena of " +
-                                                       std::to_string(total) + " bytes)"));
+  t->mem_bytes = total;
+  if (!ctx->opt.defer_tree_mem) {
+    t->mem = (char*)ctx->pool->alloc((size_t)total);
+    if (!t->mem) {
+      delete t;
+      return tree_fail(ctx, Status::err(OME_ERR_OOM, "device pool exhausted (tree arena of " +
+                                                         std::to_string(total) + " bytes)"));
+    }
   }
   // ---- info
   t->info.n_steps = nnodes;
@@ -539,6 +558,10 @@ int ome_tree_execute(ome_tree* t, const void* const* xs, void* y, const ome_scal
   for (int k = 0; k < t->n_leaves; ++k)
     if (!xv[k] && prod(t->leaf_ix[k], t->sizes) != 0)
       return tree_fail(ctx, Status::err(OME_ERR_INVALID_ARG, "null input pointer"));
+  {
+    Status m = ensure_mem(t);
+    if (!m.good()) return tree_fail(ctx, m);
+  }
   if (!ctx->opt.use_graphs || !t->cap_stream)
     return tree_fail(ctx, enqueue_eval(t, xv, y, sx, sy, ctx->stream)) ;
   // graph cache keyed by pointers and scalars
@@ -602,6 +625,10 @@ int ome_tree_execute_scalar_normalized(ome_tree* t, const void* const* xs, doubl
   OptScope os(ctx, t->opt);
   hipSetDevice(ctx->device);
   hipStream_t st = ctx->stream;
+  {
+    Status m = ensure_mem(t);
+    if (!m.good()) return tree_fail(ctx, m);
+  }
   std::vector<const void*> xv(xs, xs + t->n_leaves);
   double* work = (double*)(t->mem + t->arena_bytes + t->scratch_bytes + t->stage_bytes + t->acc_bytes);
   double* logacc = work + 1 + 1024;
@@ -768,9 +795,45 @@ int ome_tree_destroy(ome_tree* t) {
   hipStreamSynchronize(ctx->stream);
   for (auto& ge : t->graphs) hipGraphExecDestroy(ge.exec);
   if (t->cap_stream) hipStreamDestroy(t->cap_stream);
-  if (t->mem) ctx->pool->free(t->mem);
+  if (t->mem && t->owns_mem) ctx->pool->free(t->mem);
   delete t;
   return OME_OK;
 }
 
 }  // extern "C"
+
+// ---- request-loop access (csrc/server.hip) ------------------------------------------------
+
+namespace ome {
+
+int64_t tree_mem_bytes(const ome_tree* t) { return t->mem_bytes; }
+
+// Evaluate t once on st in memory lent by the caller (mem_bytes bytes): the plan's own
+// memory, if any, goes back to the pool and its graphs are dropped.  y <- result.
+Status tree_run_lent(ome_tree* t, const void* const* xs, void* y, char* mem, hipStream_t st) {
+  ome_ctx* ctx = t->ctx;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  OptScope os(ctx, t->opt);
+  hipSetDevice(ctx->device);
+  if (t->mem != mem) {
+    if (t->mem && t->owns_mem) ctx->pool->free(t->mem);
+    for (auto& ge : t->graphs) hipGraphExecDestroy(ge.exec);
+    t->graphs.clear();
+    t->mem = mem;
+    t->owns_mem = false;
+  }
+  if (prod(t->root_iy, t->sizes) == 0) return Status::ok();
+  std::vector<const void*> xv(xs, xs + t->n_leaves);
+  for (int k = 0; k < t->n_leaves; ++k)
+    if (!xv[k] && prod(t->leaf_ix[k], t->sizes) != 0)
+      return Status::err(OME_ERR_INVALID_ARG, "null input pointer");
+  return enqueue_eval(t, xv, y, scal_one(t->dt), scal_zero(t->dt), st);
+}
+
+// Forget lent memory (the lender frees it).
+void tree_return_lent(ome_tree* t) {
+  std::lock_guard<std::recursive_mutex> g(t->ctx->mu);
+  if (!t->owns_mem) t->mem = nullptr;
+}
+
+}  // namespace ome

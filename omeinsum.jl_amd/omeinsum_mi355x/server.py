@@ -1,0 +1,96 @@
+"""Continuous-batching request loop over ome_server_* (no counterpart in OMEinsum).
+
+    srv = Server(hbm_budget=8 << 30, max_active=8, backend=MI355XBackend())
+    plan = srv.plan(tree, xs, size_dict)      # device plan, memory sized but not allocated
+    t = srv.submit(plan, xs)                  # returns at once; inputs stay on the device
+    y, latency_ms = srv.wait(t)
+
+A native scheduler thread admits queued requests, in arrival order, whenever an execution
+slot is free and the plan's device memory fits the remaining budget; each slot runs on its
+own stream, so admitted requests overlap on the device and requests that arrive later are
+admitted as earlier ones retire.  Latency is measured from submission to the moment the
+scheduler sees the request's completion event.
+"""
+import ctypes
+
+from . import _lib
+from .backends import MI355XBackend
+from .device import Tropical
+from .tree import TreePlan, element_code
+
+
+class Server:
+    def __init__(self, hbm_budget=8 << 30, max_active=8, backend=None, device=None):
+        import torch
+        self.ctx = _lib.context(device)
+        self.backend = backend if backend is not None else MI355XBackend()
+        h = ctypes.c_void_p()
+        _lib.check(self.ctx.lib.ome_server_create(self.ctx.h, int(hbm_budget), int(max_active),
+                                                  ctypes.byref(h)), self.ctx.h)
+        self.h = h
+        self.budget = int(hbm_budget)
+        self._live = {}
+        self._torch = torch
+
+    def plan(self, code, xs, size_dict):
+        """A device plan of `code` (NestedEinsum or SlicedEinsum) whose memory the loop lends
+        at admission."""
+        self.backend.apply(self.ctx)
+        code_dt, _ = element_code(xs)
+        return TreePlan(code, size_dict, code_dt, self.ctx.device, defer_memory=True)
+
+    def submit(self, plan, xs, y=None):
+        """Queue one evaluation of `plan` on the device tensors xs; returns a ticket.  The
+        output (allocated when y is None) and the inputs stay referenced until wait()."""
+        self.ctx.sync_stream()
+        ts, ptrs = plan._ptrs(xs)
+        if y is None:
+            res, yt = plan._out(ts, isinstance(xs[0], Tropical))
+        else:
+            res, yt = y, (y.data if isinstance(y, Tropical) else y)
+        t = ctypes.c_int64()
+        _lib.check(self.ctx.lib.ome_server_submit(self.h, plan.h, ptrs,
+                                                  ctypes.c_void_p(yt.data_ptr()),
+                                                  ctypes.byref(t)), self.ctx.h)
+        self._live[t.value] = (plan, ts, res)
+        return t.value
+
+    def poll(self, ticket):
+        """0 queued, 1 running, 2 done, < 0 failed."""
+        st = ctypes.c_int()
+        _lib.check(self.ctx.lib.ome_server_poll(self.h, int(ticket), ctypes.byref(st)), self.ctx.h)
+        return st.value
+
+    def wait(self, ticket):
+        """Block until the request is done: (output, latency in milliseconds)."""
+        lat = ctypes.c_double()
+        try:
+            _lib.check(self.ctx.lib.ome_server_wait(self.h, int(ticket), ctypes.byref(lat)),
+                       self.ctx.h)
+        finally:
+            entry = self._live.pop(ticket, None)
+        return entry[2], lat.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.ome_server_destroy(self.h)
+            self.h = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
+def plan_bytes(plan):
+    """Device bytes the loop lends to a plan at admission."""
+    return int(plan.info().arena_bytes)
+

@@ -29,7 +29,11 @@ def tree_desc(ne, ids):
 
 
 class TreePlan:
-    def __init__(self, code, size_dict, code_dt, device):
+    """A device plan of one tree.  defer_memory=True sizes the plan's device memory without
+    allocating it (the request loop lends memory when it admits the plan; a direct call
+    allocates it on first use)."""
+
+    def __init__(self, code, size_dict, code_dt, device, defer_memory=False):
         if isinstance(code, SlicedEinsum):
             ne, slicing = code.eins, list(code.slicing)
         else:
@@ -60,9 +64,15 @@ class TreePlan:
             sizes[i] = int(size_dict[l])
         csl = (ctypes.c_int32 * max(len(slicing), 1))(*[ids[l] for l in slicing])
         h = ctypes.c_void_p()
-        _lib.check(self.ctx.lib.ome_tree_create(self.ctx.h, code_dt, cdesc, len(desc), len(ids),
-                                                sizes, len(slicing), csl, ctypes.byref(h)),
-                   self.ctx.h)
+        if defer_memory:
+            self.ctx.set_option(_lib.OPT_TREE_DEFER_MEMORY, 1)
+        try:
+            _lib.check(self.ctx.lib.ome_tree_create(self.ctx.h, code_dt, cdesc, len(desc),
+                                                    len(ids), sizes, len(slicing), csl,
+                                                    ctypes.byref(h)), self.ctx.h)
+        finally:
+            if defer_memory:
+                self.ctx.set_option(_lib.OPT_TREE_DEFER_MEMORY, 0)
         self.h = h
 
     def __del__(self):
@@ -140,7 +150,8 @@ _CACHE = collections.OrderedDict()
 _CACHE_MAX = 64
 
 
-def plan_for(code, xs, size_dict):
+def element_code(xs):
+    """The C ABI element type of a tree's inputs (they must share one type)."""
     trop = isinstance(xs[0], Tropical)
     ts = [x.data if isinstance(x, Tropical) else to_device(x) for x in xs]
     import torch
@@ -149,7 +160,11 @@ def plan_for(code, xs, size_dict):
         T = torch.promote_types(T, t.dtype)
     if any(t.dtype != T for t in ts):
         raise TypeError("tree inputs must share one element type (promote first)")
-    code_dt = dtype_code(T, trop)
+    return dtype_code(T, trop), ts
+
+
+def plan_for(code, xs, size_dict):
+    code_dt, ts = element_code(xs)
     labels = code.uniquelabels() if not isinstance(code, SlicedEinsum) else code.eins.uniquelabels()
     # a plan freezes the context's planning options when it is created
     ctx = _lib.context(ts[0].device.index)

@@ -1,0 +1,115 @@
+"""GPU tests of the continuous-batching request loop (ome_server_*) through the C ABI:
+results through the loop equal the oracle and equal the same request evaluated alone, when
+requests of different sizes are mixed, queued behind the HBM budget and the slot count, and
+sliced."""
+import numpy as np
+import pytest
+
+from oracle import default_backend as db
+from omeinsum_mi355x import codes, contractionorder as co, networks
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    yield
+
+
+def om():
+    import omeinsum_mi355x
+    return omeinsum_mi355x
+
+
+def request(n, seed, dtype=np.complex64):
+    edges = networks.random_regular_graph(n, 3, seed=seed)
+    code = networks.network_code(n, edges)
+    sd = {e: 2 for e in range(len(edges))}
+    xs = networks.gaussian_tensors(code, sd, seed + 1, dtype)
+    tree = co.optimize_code(code, sd)
+    return tree, sd, xs, [torch.from_numpy(x).cuda() for x in xs]
+
+
+def fp64(tree, sd, xs):
+    return complex(db.nested_einsum(tree.to_plain(), [x.astype(np.complex128) for x in xs], sd)[()])
+
+
+def test_single_request_matches_oracle_and_direct_plan():
+    from omeinsum_mi355x.server import Server
+    tree, sd, xs, dxs = request(40, 5001)
+    ref = fp64(tree, sd, xs)
+    with Server(hbm_budget=1 << 30, max_active=2) as srv:
+        plan = srv.plan(tree, dxs, sd)
+        y, lat = srv.wait(srv.submit(plan, dxs))
+        got = complex(om().to_numpy(y)[()])
+        assert abs(got - ref) <= 1e-4 * abs(ref), (got, ref)
+        assert lat > 0
+        # the same plan called directly (it allocates its own memory on first use)
+        direct = complex(om().to_numpy(plan.execute(dxs))[()])
+        assert direct == got
+
+
+def test_mixed_requests_equal_alone():
+    # requests of different sizes: each result through a busy loop (4 slots, a budget that
+    # holds only a few plans at a time) equals the same request run alone
+    from omeinsum_mi355x.server import Server, plan_bytes
+    reqs = [request(n, 5100 + i) for i, n in enumerate([20, 44, 30, 60, 24, 52, 36, 28, 48, 40,
+                                                         22, 56, 32, 26, 46, 38])]
+    with Server(hbm_budget=1 << 30, max_active=1) as alone_srv:
+        alone = []
+        for tree, sd, xs, dxs in reqs:
+            plan = alone_srv.plan(tree, dxs, sd)
+            y, _ = alone_srv.wait(alone_srv.submit(plan, dxs))
+            alone.append(complex(om().to_numpy(y)[()]))
+    plans = []
+    with Server(hbm_budget=1 << 30, max_active=4) as probe:
+        plans = [probe.plan(tree, dxs, sd) for tree, sd, xs, dxs in reqs]
+    budget = 3 * max(plan_bytes(p) for p in plans)
+    with Server(hbm_budget=budget, max_active=4) as srv:
+        plans = [srv.plan(tree, dxs, sd) for tree, sd, xs, dxs in reqs]
+        tickets = [srv.submit(p, r[3]) for p, r in zip(plans, reqs)]
+        states = [srv.poll(t) for t in tickets]
+        assert all(s in (0, 1, 2) for s in states)
+        mixed = [complex(om().to_numpy(srv.wait(t)[0])[()]) for t in tickets]
+    for (tree, sd, xs, _), a, m in zip(reqs, alone, mixed):
+        assert abs(m - a) <= 1e-6 * abs(a), (m, a)
+        ref = fp64(tree, sd, xs)
+        assert abs(m - ref) <= 1e-4 * abs(ref), (m, ref)
+
+
+def test_budget_admission_and_rejection():
+    from omeinsum_mi355x import _lib
+    from omeinsum_mi355x.server import Server, plan_bytes
+    tree, sd, xs, dxs = request(48, 5201)
+    with Server(hbm_budget=1 << 30, max_active=2) as probe:
+        need = plan_bytes(probe.plan(tree, dxs, sd))
+    # a budget that holds exactly one copy: eight submissions run one after another
+    with Server(hbm_budget=need, max_active=4) as srv:
+        plans = [srv.plan(tree, dxs, sd) for _ in range(8)]
+        tickets = [srv.submit(p, dxs) for p in plans]
+        outs = [complex(om().to_numpy(srv.wait(t)[0])[()]) for t in tickets]
+        assert len(set(outs)) == 1
+    # a plan larger than the whole budget is refused at submission
+    with Server(hbm_budget=max(need // 2, 1), max_active=2) as srv:
+        plan = srv.plan(tree, dxs, sd)
+        with pytest.raises(MemoryError):
+            srv.submit(plan, dxs)
+    assert isinstance(_lib.OPT_TREE_DEFER_MEMORY, int)
+
+
+def test_sliced_request_through_the_loop():
+    from omeinsum_mi355x.server import Server
+    tree, sd, xs, dxs = request(40, 5301)
+    sliced = co.slice_code(tree, sd, sc_target=5)
+    assert sliced.slicing
+    ref = fp64(tree, sd, xs)
+    with Server(hbm_budget=1 << 30, max_active=2) as srv:
+        p1, p2 = srv.plan(tree, dxs, sd), srv.plan(sliced, dxs, sd)
+        t1, t2 = srv.submit(p1, dxs), srv.submit(p2, dxs)
+        a = complex(om().to_numpy(srv.wait(t1)[0])[()])
+        b = complex(om().to_numpy(srv.wait(t2)[0])[()])
+    assert abs(a - ref) <= 1e-4 * abs(ref)
+    assert abs(b - ref) <= 1e-4 * abs(ref)
