@@ -18,7 +18,7 @@ by the MI355X backend (graph replay) next to the plain per-step GPU path; 4096 u
 GEMMs and 1024 permutes in two grouped launches next to one call per problem; the CPU
 baseline (the oracle's numpy restatement of OMEinsum's DefaultBackend on this host).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|c4|c3]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|c4|c3|c5]
                        [--no-cpu] [--copy-calibration]
 """
 import argparse
@@ -280,6 +280,106 @@ def bench_c3(steps, warmup):
     return out
 
 
+def bench_c5(nreq=48, sc_budget=22, hbm_budget=16 << 30, max_active=8):
+    """Serving stream: requests are random 3-regular graphs with an even number of vertices in
+    [40, 140], bond dimension 2, ComplexF32 Gaussian tensors, greedy order; a request whose
+    largest intermediate exceeds 2^sc_budget elements is sliced (FP64 cross-slice sum).
+    Inputs and plans are prepared before the timed runs (planning time reported apart).
+    Measured: the plain per-step path back to back, the request loop at saturation (every
+    request submitted at once), and the loop under Poisson arrivals at 70% of its saturation
+    rate (p50 / p99 latency from submission to completion)."""
+    import numpy as np
+    import torch
+    import omeinsum_mi355x as om
+    from omeinsum_mi355x import contractionorder as co, networks
+    from omeinsum_mi355x.server import Server
+
+    r = np.random.Generator(np.random.PCG64(5000))
+    reqs = []
+    t_plan = time.perf_counter()
+    flops = 0.0
+    for _ in range(nreq):
+        n = 2 * int(r.integers(20, 71))
+        edges = networks.random_regular_graph(n, 3, seed=int(r.integers(1 << 30)))
+        code = networks.network_code(n, edges)
+        sd = {e: 2 for e in range(len(edges))}
+        xs = networks.gaussian_tensors(code, sd, int(r.integers(1 << 30)), np.complex64)
+        tree = co.optimize_code(code, sd)
+        if co.contraction_complexity(tree, sd)[1] > sc_budget:
+            tree = co.slice_code(tree, sd, sc_target=sc_budget)
+        f = 8.0 * co.flop(tree, sd)
+        flops += f
+        reqs.append((n, tree, sd, [torch.from_numpy(x).cuda() for x in xs], f))
+    t_plan = time.perf_counter() - t_plan
+    sliced = sum(1 for q in reqs if hasattr(q[1], "slicing") and q[1].slicing)
+    out = {"requests": nreq, "sliced": sliced, "sc_budget_log2_elems": sc_budget,
+           "total_flop": flops, "host_planning_s": t_plan}
+
+    def scalar(y):
+        return complex(om.to_numpy(y)[()])
+
+    # 1. the plain per-step path (one einsum per tree node from the host, new output per node)
+    om.set_einsum_backend(om.DefaultBackend())
+    om.einsum(reqs[0][1], reqs[0][3], reqs[0][2])
+    torch.cuda.synchronize()
+    per_step = []
+    lat = []
+    t0 = time.perf_counter()
+    for n, tree, sd, dxs, f in reqs:
+        a = time.perf_counter()
+        per_step.append(scalar(om.einsum(tree, dxs, sd)))
+        lat.append((time.perf_counter() - a) * 1e3)
+    t_ps = time.perf_counter() - t0
+    out["per_step"] = {"req_per_s": nreq / t_ps, "p50_ms": float(np.percentile(lat, 50)),
+                       "p99_ms": float(np.percentile(lat, 99)), "tflops": flops / t_ps / 1e12}
+    log("C5 per-step path: %.1f req/s (p50 %.2f ms, p99 %.2f ms)" %
+        (nreq / t_ps, out["per_step"]["p50_ms"], out["per_step"]["p99_ms"]))
+
+    with Server(hbm_budget=hbm_budget, max_active=max_active,
+                backend=om.MI355XBackend(precision="fp32")) as srv:
+        t0 = time.perf_counter()
+        plans = [srv.plan(tree, dxs, sd) for n, tree, sd, dxs, f in reqs]
+        out["device_planning_s"] = time.perf_counter() - t0
+        # warm-up pass
+        for t in [srv.submit(p, q[3]) for p, q in zip(plans, reqs)]:
+            srv.wait(t)
+        # 2. saturation: every request queued at once
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tickets = [srv.submit(p, q[3]) for p, q in zip(plans, reqs)]
+        res = [srv.wait(t) for t in tickets]
+        t_sat = time.perf_counter() - t0
+        sat_lat = [l for _, l in res]
+        loop_vals = [scalar(y) for y, _ in res]
+        sat_rps = nreq / t_sat
+        out["loop_saturated"] = {"req_per_s": sat_rps, "tflops": flops / t_sat / 1e12,
+                                 "p50_ms": float(np.percentile(sat_lat, 50)),
+                                 "p99_ms": float(np.percentile(sat_lat, 99))}
+        # 3. Poisson arrivals at 70% of the saturated rate
+        rate = 0.7 * sat_rps
+        gaps = np.random.Generator(np.random.PCG64(5001)).exponential(1.0 / rate, nreq)
+        arrivals = np.cumsum(gaps)
+        tickets = []
+        t0 = time.perf_counter()
+        for p, q, ta in zip(plans, reqs, arrivals):
+            while time.perf_counter() - t0 < ta:
+                pass
+            tickets.append(srv.submit(p, q[3]))
+        res = [srv.wait(t) for t in tickets]
+        t_poi = time.perf_counter() - t0
+        lats = [l for _, l in res]
+        out["loop_poisson"] = {"offered_req_per_s": rate, "req_per_s": nreq / t_poi,
+                               "p50_ms": float(np.percentile(lats, 50)),
+                               "p99_ms": float(np.percentile(lats, 99))}
+    rel = [abs(a - b) / abs(b) for a, b in zip(loop_vals, per_step)]
+    out["max_rel_diff_loop_vs_per_step"] = max(rel)
+    out["speedup_req_per_s"] = sat_rps / out["per_step"]["req_per_s"]
+    log("C5 request loop: %.1f req/s saturated (%.2fx the per-step path); Poisson at %.1f req/s: "
+        "p50 %.2f ms, p99 %.2f ms" % (sat_rps, out["speedup_req_per_s"], rate,
+                                      out["loop_poisson"]["p50_ms"], out["loop_poisson"]["p99_ms"]))
+    return out
+
+
 def copy_calibration(reps=5):
     """A streaming copy of 1 GiB through one elementwise kernel (y = x + 0: each element read
     once, written once), repeated: the reading of the FETCH_SIZE / WRITE_SIZE counters for a
@@ -335,7 +435,7 @@ def main():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", choices=("all", "c4", "c3"), default="all")
+    p.add_argument("--workload", choices=("all", "c4", "c3", "c5"), default="all")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--copy-calibration", action="store_true")
     args = p.parse_args()
@@ -357,6 +457,8 @@ def main():
         extra["c2_network_100"] = bench_c2(max(5, args.steps // 2), 2)
     if args.workload in ("all", "c3"):
         extra["c3_grouped"] = bench_c3(max(3, args.steps // 4), 1)
+    if args.workload in ("all", "c5"):
+        extra["c5_serving"] = bench_c5()
     if args.copy_calibration:
         copy_calibration()
     cpu = None if args.no_cpu else cpu_baseline(ab)
@@ -388,6 +490,15 @@ def main():
         "cpu_baseline": cpu,
         "extra": extra,
     }
+    c5 = extra.get("c5_serving")
+    if c5:
+        # the tensor-network request stream: request loop vs the plain per-step GPU path
+        line["tn_requests"] = {"req_per_s": round(c5["loop_saturated"]["req_per_s"], 1),
+                               "per_step_req_per_s": round(c5["per_step"]["req_per_s"], 1),
+                               "vs_per_step": round(c5["speedup_req_per_s"], 2),
+                               "p50_ms": round(c5["loop_poisson"]["p50_ms"], 3),
+                               "p99_ms": round(c5["loop_poisson"]["p99_ms"], 3),
+                               "offered_req_per_s": round(c5["loop_poisson"]["offered_req_per_s"], 1)}
     print(json.dumps(line), flush=True)
 
 
