@@ -343,20 +343,23 @@ __global__ __launch_bounds__(256) void k_gemm_packed_bf16x3(PackedGemmDesc d) {
 //   issue the copy of kt+2 into the stage kt just vacated
 //   read s0(kt+1) | MFMAs of s1(kt)
 // so LDS reads are always in flight under MFMAs and there is one barrier per K-step.
-template <int TM, int TN>
-__global__ __launch_bounds__(256) void k_gemm_packed_bf16x3_pipe(PackedGemmDesc d) {
-  constexpr int MB = TM / 64, NB = TN / 64;
+// WM x WN waves (64 * WM * WN threads); each wave owns a (TM/WM) x (TN/WN) block of 32x32
+// accumulators.
+template <int TM, int TN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_packed_bf16x3_pipe(PackedGemmDesc d) {
+  constexpr int NW = WM * WN;
+  constexpr int MB = TM / WM / 32, NB = TN / WN / 32;
   constexpr int A_BYTES = TM * PK_K * 2 * 2, B_BYTES = TN * PK_K * 2 * 2;
   constexpr int STAGE = A_BYTES + B_BYTES;
   constexpr int CHUNKS = STAGE / 1024;
-  constexpr int CPW = CHUNKS / 4;
-  static_assert(CHUNKS % 4 == 0, "copies must split evenly over 4 waves");
+  constexpr int CPW = CHUNKS / NW;
+  static_assert(CHUNKS % NW == 0, "copies must split evenly over the waves");
   __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * STAGE];
   int64_t tm, tn, batch, ks;
   pk_map(blockIdx.x, gridDim.x, d, tm, tn, batch, ks);
   const int t = threadIdx.x;
   const int lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
-  const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
+  const int wm = (w % WM) * (TM / WM), wn = (w / WM) * (TN / WN);
   const int64_t kt0 = ks * d.kt_per_split;
   int64_t kt1 = kt0 + d.kt_per_split;
   if (kt1 > d.tiles_k) kt1 = d.tiles_k;
@@ -534,8 +537,8 @@ bool packed_plain_overwrite(const GemmDesc& g) {
   return g.sx.re == 1.0 && g.sx.im == 0.0 && g.sy.re == 0.0 && g.sy.im == 0.0;
 }
 
-// Workgroup tile (TM x TN) for a packed GEMM.  OME_PACKED_TILE=128|256x128|128x256|256
-// forces one for measurements; by default large products use 256x128.
+// Workgroup tile (TM x TN) for a packed GEMM.  OME_PACKED_TILE=128|256x128|128x256|256|256x256
+// forces one for measurements.
 static void packed_tiles(int64_t M, int64_t N, int& TM, int& TN) {
   static int forced = -1;
   if (forced < 0) {
@@ -546,6 +549,7 @@ static void packed_tiles(int64_t M, int64_t N, int& TM, int& TN) {
       else if (!strcmp(e, "256x128")) forced = 2;
       else if (!strcmp(e, "128x256")) forced = 3;
       else if (!strcmp(e, "256")) forced = 4;
+      else if (!strcmp(e, "256x256")) forced = 5;
     }
   }
   switch (forced) {
@@ -553,6 +557,14 @@ static void packed_tiles(int64_t M, int64_t N, int& TM, int& TN) {
     case 2: TM = 256; TN = 128; return;
     case 3: TM = 128; TN = 256; return;
     case 4: TM = 256; TN = 128; return;
+    case 5: TM = 256; TN = 256; return;
+  }
+  // 256 x 256 workgroup tiles (8 waves of 128 x 64) halve the LDS traffic per MFMA; use them
+  // when they still give at least half a tile per CU, else 128 x 128
+  if (((M + 255) / 256) * ((N + 255) / 256) >= 128) {
+    TM = 256;
+    TN = 256;
+    return;
   }
   TM = 128;
   TN = 128;
@@ -653,8 +665,10 @@ void launch_gemm_packed(int dt, const GemmDesc& g, void* pa, void* pb, float* wo
   } else if (TN == 256 && TM == 128) {
     if (stages != 2) hipLaunchKernelGGL((k_gemm_packed_bf16x3<128, 256, 1>), grid, blk, 0, st, d);
     else hipLaunchKernelGGL((k_gemm_packed_bf16x3<128, 256, 2>), grid, blk, 0, st, d);
+  } else if (TM == 256 && TN == 256) {
+    hipLaunchKernelGGL((k_gemm_packed_bf16x3_pipe<256, 256, 2, 4>), grid, dim3(512), 0, st, d);
   } else if (stages == 4) {
-    hipLaunchKernelGGL((k_gemm_packed_bf16x3_pipe<128, 128>), grid, blk, 0, st, d);
+    hipLaunchKernelGGL((k_gemm_packed_bf16x3_pipe<128, 128, 2, 2>), grid, blk, 0, st, d);
   } else {
     if (stages == 1) hipLaunchKernelGGL((k_gemm_packed_bf16x3<128, 128, 1>), grid, blk, 0, st, d);
     else if (stages == 3) hipLaunchKernelGGL((k_gemm_packed_bf16x3<128, 128, 3>), grid, blk, 0, st, d);

@@ -361,6 +361,26 @@ def test_tropical_gemm_exact(dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.complex64])
+@pytest.mark.parametrize("spec", ["ij,jk->ik", "ji,kj->ki"])
+def test_packed_bf16x3_256_tiles(dtype, spec):
+    # products with >= 128 tiles of 256 x 256 take the 8-wave 256 x 256 kernel; ragged edges
+    o = om()
+    ixs, iy = parse(spec)
+    sd = {"i": 2900, "j": 520, "k": 2900}
+    r = rng(1415)
+    xs = [rand(r, [sd[l] for l in ix], dtype) for ix in ixs]
+    o.set_einsum_backend(o.MI355XBackend(precision="bf16x3"))
+    try:
+        got = run(ixs, iy, xs, sd)
+    finally:
+        o.set_einsum_backend(o.DefaultBackend())
+    W = wide(dtype)
+    a, b = (x.astype(W) for x in xs)
+    ref = (a @ b) if spec == "ij,jk->ik" else (b @ a)
+    assert_close(got, ref, dtype, 16 * 520, spec)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
 @pytest.mark.parametrize("spec", ["ij,jk->ik", "ji,kj->ik", "ijb,jkb->ikb", "ajb,bkj->bka", "ij,jk->ki"])
 @pytest.mark.parametrize("scal", [(True, False), (1.5 - 0.5j, 0.25)])
 def test_packed_bf16x3(dtype, spec, scal):
