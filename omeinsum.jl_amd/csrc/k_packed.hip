@@ -79,7 +79,45 @@ __global__ __launch_bounds__(256) void k_pack_bf16x3(PackDesc d) {
   const float* base = (const float*)d.src + batch * d.sb * (d.mode ? 2 : 1);
   const int64_t r0 = tr * TR, k0 = tk * PK_K;
   const int t = threadIdx.x;
-  if (d.rfast) {
+  // fast path: real operand, a unit-stride dim, a full tile and 16-byte aligned runs:
+  // TR*32/1024 float4 loads per thread, all issued before the LDS stores
+  constexpr int NV = TR * PK_K / 1024;
+  const bool full = d.mode == 0 && r0 + TR <= d.R && k0 + PK_K <= d.K;
+  if (full && d.rfast && d.sr == 1 && (d.sk % 4) == 0 && (((uintptr_t)base) & 15) == 0) {
+    float4 v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int e = t + 256 * i;
+      const int r4 = (e % (TR / 4)) * 4, k = e / (TR / 4);
+      v[i] = *(const float4*)(base + (r0 + r4) + (k0 + k) * d.sk);
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int e = t + 256 * i;
+      const int r4 = (e % (TR / 4)) * 4, k = e / (TR / 4);
+      tile[r4][k] = v[i].x;
+      tile[r4 + 1][k] = v[i].y;
+      tile[r4 + 2][k] = v[i].z;
+      tile[r4 + 3][k] = v[i].w;
+    }
+  } else if (full && d.sk == 1 && (d.sr % 4) == 0 && (((uintptr_t)base) & 15) == 0) {
+    float4 v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int e = t + 256 * i;
+      const int k4 = (e % (PK_K / 4)) * 4, r = e / (PK_K / 4);
+      v[i] = *(const float4*)(base + (r0 + r) * d.sr + k0 + k4);
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int e = t + 256 * i;
+      const int k4 = (e % (PK_K / 4)) * 4, r = e / (PK_K / 4);
+      tile[r][k4] = v[i].x;
+      tile[r][k4 + 1] = v[i].y;
+      tile[r][k4 + 2] = v[i].z;
+      tile[r][k4 + 3] = v[i].w;
+    }
+  } else if (d.rfast) {
     // lanes along rows
 #pragma unroll 4
     for (int i = 0; i < TR / 8; ++i) {
