@@ -130,6 +130,72 @@ def bench_c4(steps, warmup, precs=("bf16x3", "fp32")):
     return out, (a, b)
 
 
+def bench_c4_more(ab, steps, warmup):
+    """The rest of the 64^4 contraction's configuration: Float32 at FP64 precision (widened
+    operands on the FP64 matrix cores), ComplexF32 inputs (realified into one real GEMM) in
+    BF16x3 and FP32, and forward + backward for both inputs (the ChainRules rule of
+    src/autodiff.jl:24-33) in BF16x3, with the gradients checked against FP64 on sampled rows."""
+    import numpy as np
+    import torch
+    import omeinsum_mi355x as om
+
+    a, b = ab
+    code = om.ein("abcd,cdef->abef")
+    sd = om.get_size_dict(code.ixs, (a, b))
+    out = {}
+    y = torch.empty(64, 64, 64, 64, device="cuda")
+    om.set_einsum_backend(om.MI355XBackend(precision="fp64"))
+    ms = time_events(lambda: om.einsum_inplace(code, (a, b), y, True, False, sd), max(3, steps // 4), 1)
+    out["f32_at_fp64"] = {"ms": ms, "tflops": C4_FLOPS / ms / 1e9,
+                          "frac_fp64_peak": C4_FLOPS / ms / 1e9 / PEAK_FP64}
+    g = torch.Generator(device="cuda").manual_seed(4001)
+    ac = torch.complex(torch.randn(64, 64, 64, 64, device="cuda", generator=g),
+                       torch.randn(64, 64, 64, 64, device="cuda", generator=g)) * (0.5 ** 0.5)
+    bc = torch.complex(torch.randn(64, 64, 64, 64, device="cuda", generator=g),
+                       torch.randn(64, 64, 64, 64, device="cuda", generator=g)) * (0.5 ** 0.5)
+    yc = torch.empty(64, 64, 64, 64, dtype=torch.complex64, device="cuda")
+    an = ac.cpu().numpy().astype(np.complex128).reshape(4096, 4096)
+    bn = bc.cpu().numpy().astype(np.complex128).reshape(4096, 4096)
+    rows = np.arange(0, 4096, 128)
+    ref = an[rows] @ bn
+    for prec in ("bf16x3", "fp32"):
+        om.set_einsum_backend(om.MI355XBackend(precision=prec))
+        fn = lambda: om.einsum_inplace(code, (ac, bc), yc, True, False, sd)
+        ms = time_events(fn, max(3, steps // 4), 1)
+        fn()
+        got = yc.cpu().numpy().reshape(4096, 4096)[rows].astype(np.complex128)
+        err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        out["c32_" + prec] = {"ms": ms, "tflops": 4 * C4_FLOPS / ms / 1e9, "rel_err_fp64": err}
+    del ac, bc, yc, an, bn
+    # forward + backward in BF16x3
+    om.set_einsum_backend(om.MI355XBackend(precision="bf16x3"))
+    dy = torch.randn(64, 64, 64, 64, device="cuda", generator=g)
+    res = {}
+
+    def fwd_bwd():
+        yy = om.einsum(code, (a, b), sd)
+        res["g"] = om.einsum_backward_rule(code, (a, b), yy, sd, dy)
+
+    ms = time_events(fwd_bwd, max(3, steps // 4), 1)
+    fwd_bwd()
+    torch.cuda.synchronize()
+    a2 = a.cpu().numpy().astype(np.float64).reshape(4096, 4096)
+    b2 = b.cpu().numpy().astype(np.float64).reshape(4096, 4096)
+    d2 = dy.cpu().numpy().astype(np.float64).reshape(4096, 4096)
+    da = res["g"][0].cpu().numpy().reshape(4096, 4096)[rows].astype(np.float64)
+    db_ = res["g"][1].cpu().numpy().reshape(4096, 4096)[rows].astype(np.float64)
+    ra = d2[rows] @ b2.T
+    rb = a2[:, rows].T @ d2
+    out["forward_backward_bf16x3"] = {
+        "ms": ms, "tflops": 3 * C4_FLOPS / ms / 1e9,
+        "grad_a_rel_err_fp64": float(np.linalg.norm(da - ra) / np.linalg.norm(ra)),
+        "grad_b_rel_err_fp64": float(np.linalg.norm(db_ - rb) / np.linalg.norm(rb))}
+    om.set_einsum_backend(om.DefaultBackend())
+    log("C4 more: %s" % {k: {q: ("%.3g" % w if isinstance(w, float) else w) for q, w in v.items()}
+                         for k, v in out.items()})
+    return out
+
+
 def bench_c1(steps, warmup):
     import torch
     import omeinsum_mi355x as om
@@ -280,10 +346,11 @@ def bench_c3(steps, warmup):
     return out
 
 
-def bench_c5(nreq=48, sc_budget=22, hbm_budget=16 << 30, max_active=8):
+def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
     """Serving stream: requests are random 3-regular graphs with an even number of vertices in
     [40, 140], bond dimension 2, ComplexF32 Gaussian tensors, greedy order; a request whose
-    largest intermediate exceeds 2^sc_budget elements is sliced (FP64 cross-slice sum).
+    largest intermediate exceeds 2^sc_budget elements (2^25 ComplexF32 = 256 MiB, a plan of
+    about 1 GiB) is sliced to fit (FP64 cross-slice sum).
     Inputs and plans are prepared before the timed runs (planning time reported apart).
     Measured: the plain per-step path back to back, the request loop at saturation (every
     request submitted at once), and the loop under Poisson arrivals at 70% of its saturation
@@ -453,6 +520,7 @@ def main():
     c4, ab = bench_c4(args.steps, args.warmup, precs)
     extra = {"c4": c4}
     if args.workload == "all":
+        extra["c4_more"] = bench_c4_more(ab, args.steps, args.warmup)
         extra["c1_ijk_jkl_il_64"] = bench_c1(args.steps, args.warmup)
         extra["c2_network_100"] = bench_c2(max(5, args.steps // 2), 2)
     if args.workload in ("all", "c3"):

@@ -33,14 +33,16 @@ def _conj_copy(t):
 
 
 def einsum_grad(ixs, xs, iy, size_dict, dy, i):
-    """src/autodiff.jl:24-30."""
+    """src/autodiff.jl:24-30.  conj is the identity on real tensors, so real operands are
+    used as they are instead of copied."""
     from .ein_ops import flat_einsum
     nixs = [list(ix) for ix in ixs]
     nixs[i] = list(iy)
     nxs = [to_device(x) for x in xs]
-    nxs[i] = _conj_copy(to_device(dy))
+    d = to_device(dy)
+    nxs[i] = _conj_copy(d) if d.is_complex() else d
     y = flat_einsum(EinCode(nixs, list(ixs[i])), tuple(nxs), size_dict)
-    g = _conj_copy(y)
+    g = _conj_copy(y) if y.is_complex() else y
     x = to_device(xs[i])
     if not x.is_complex() and g.is_complex():
         g = g.real.contiguous()  # ProjectTo(x) keeps the real part

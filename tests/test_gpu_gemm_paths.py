@@ -136,3 +136,57 @@ def test_few_outputs_long_k(spec, sd, dtype):
                 assert np.array_equal(got, ref), spec
             else:
                 assert_close(got, ref, dt, 4 * k, spec + " scaled")
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
+def test_c4_shaped_backward_rule(prec, dtype):
+    # forward + the ChainRules rule (src/autodiff.jl:24-33) of abcd,cdef->abef at size 16
+    # (M = N = K = 256: the matrix-core paths) against the oracle's einsum_grad in FP64
+    from oracle import default_backend as db
+    o = om()
+    ixs, iy = parse("abcd,cdef->abef")
+    sd = {l: 16 for l in "abcdef"}
+    r = rng(2104)
+    xs = [rand(r, [16] * 4, dtype) for _ in range(2)]
+    dy = rand(r, [16] * 4, dtype)
+    o.set_einsum_backend(o.MI355XBackend(precision=prec))
+    try:
+        code = o.EinCode(ixs, iy)
+        dxs = [dev(x) for x in xs]
+        y = o.einsum(code, dxs, sd)
+        grads = o.einsum_backward_rule(code, dxs, y, sd, dev(dy))
+        torch.cuda.synchronize()
+        got = [o.to_numpy(g) for g in grads]
+    finally:
+        o.set_einsum_backend(o.DefaultBackend())
+    W = wide(dtype)
+    for i in range(2):
+        ref = db.einsum_grad(ixs, [x.astype(W) for x in xs], iy, sd, dy.astype(W), i)
+        assert_close(got[i], ref, dtype, 16 * 4096, "grad %d" % i)
+
+
+def test_c4_shaped_gradient_finite_differences():
+    # directional central differences of L(A, B) = sum(dy * (A . B)) in FP64 on the device
+    o = om()
+    ixs, iy = parse("abcd,cdef->abef")
+    sd = {l: 6 for l in "abcdef"}
+    r = rng(2105)
+    a, b, dy, va, vb = (rand(r, [6] * 4, np.float64) for _ in range(5))
+    o.set_einsum_backend(o.MI355XBackend(precision="fp64"))
+    try:
+        code = o.EinCode(ixs, iy)
+
+        def loss(x, z):
+            yy = o.to_numpy(o.einsum(code, (dev(x), dev(z)), sd))
+            return float(np.sum(dy * yy))
+
+        y = o.einsum(code, (dev(a), dev(b)), sd)
+        ga, gb = (o.to_numpy(g) for g in o.einsum_backward_rule(code, (dev(a), dev(b)), y, sd, dev(dy)))
+        h = 1e-4
+        fa = (loss(a + h * va, b) - loss(a - h * va, b)) / (2 * h)
+        fb = (loss(a, b + h * vb) - loss(a, b - h * vb)) / (2 * h)
+    finally:
+        o.set_einsum_backend(o.DefaultBackend())
+    assert abs(fa - np.sum(ga * va)) <= 1e-7 * max(1.0, abs(fa))
+    assert abs(fb - np.sum(gb * vb)) <= 1e-7 * max(1.0, abs(fb))
