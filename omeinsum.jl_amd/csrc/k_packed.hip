@@ -70,9 +70,7 @@ __device__ __forceinline__ float pack_fetch(const PackDesc& d, const float* __re
 
 // TR rows per packed tile (128 or 256).
 template <int TR>
-__global__ __launch_bounds__(256) void k_pack_bf16x3(PackDesc d) {
-  __shared__ float tile[TR][PK_K + 1];
-  int64_t bid = blockIdx.x;
+__device__ __forceinline__ void pack_tile(const PackDesc& d, int64_t bid, float (*tile)[PK_K + 1]) {
   const int64_t tk = bid % d.tiles_k; bid /= d.tiles_k;
   const int64_t tr = bid % d.tiles_r; bid /= d.tiles_r;
   const int64_t batch = bid;
@@ -157,6 +155,20 @@ __global__ __launch_bounds__(256) void k_pack_bf16x3(PackDesc d) {
     *(uint4*)(out + slot * 8) = vh;
     *(uint4*)(out + TR * PK_K + slot * 8) = vl;
   }
+}
+
+template <int TR>
+__global__ __launch_bounds__(256) void k_pack_bf16x3(PackDesc d) {
+  __shared__ float tile[TR][PK_K + 1];
+  pack_tile<TR>(d, blockIdx.x, tile);
+}
+
+// Both operands of one GEMM in one launch: blocks [0, na) pack a, the rest pack b.
+template <int TR>
+__global__ __launch_bounds__(256) void k_pack2_bf16x3(PackDesc a, PackDesc b, int64_t na) {
+  __shared__ float tile[TR][PK_K + 1];
+  if ((int64_t)blockIdx.x < na) pack_tile<TR>(a, blockIdx.x, tile);
+  else pack_tile<TR>(b, (int64_t)blockIdx.x - na, tile);
 }
 
 // ---- GEMM on packed operands ---------------------------------------------------------------
@@ -624,9 +636,8 @@ int64_t packed_tiles_mn(int64_t M, int64_t N) {
   return ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
 }
 
-static void launch_pack(const void* src, uint16_t* dst, int64_t R, int64_t K, int64_t B,
-                        int64_t sr, int64_t sk, int64_t sb, int mode, int64_t R0, int64_t K0,
-                        int TR, hipStream_t st) {
+static PackDesc pack_desc(const void* src, uint16_t* dst, int64_t R, int64_t K, int64_t sr,
+                          int64_t sk, int64_t sb, int mode, int64_t R0, int64_t K0, int TR) {
   PackDesc p{};
   p.src = src;
   p.dst = dst;
@@ -641,25 +652,44 @@ static void launch_pack(const void* src, uint16_t* dst, int64_t R, int64_t K, in
   p.tiles_r = (R + TR - 1) / TR;
   p.tiles_k = (K + PK_K - 1) / PK_K;
   p.rfast = (sr == 1 && sk != 1) ? 1 : 0;
+  return p;
+}
+
+static void launch_pack(const PackDesc& p, int64_t B, int TR, hipStream_t st) {
   const int64_t blocks = p.tiles_r * p.tiles_k * B;
   TimedScope ts(KC_PACK, st);
   if (TR == 256) hipLaunchKernelGGL(k_pack_bf16x3<256>, dim3((uint32_t)blocks), dim3(256), 0, st, p);
   else hipLaunchKernelGGL(k_pack_bf16x3<128>, dim3((uint32_t)blocks), dim3(256), 0, st, p);
 }
 
+static void launch_pack2(const PackDesc& a, const PackDesc& b, int64_t B, int TR, hipStream_t st) {
+  const int64_t na = a.tiles_r * a.tiles_k * B, nb = b.tiles_r * b.tiles_k * B;
+  TimedScope ts(KC_PACK, st);
+  if (TR == 256)
+    hipLaunchKernelGGL(k_pack2_bf16x3<256>, dim3((uint32_t)(na + nb)), dim3(256), 0, st, a, b, na);
+  else
+    hipLaunchKernelGGL(k_pack2_bf16x3<128>, dim3((uint32_t)(na + nb)), dim3(256), 0, st, a, b, na);
+}
+
 // Real F32 or complex C32 GEMM on packed BF16x3 operands.  pa/pb: workspace of
 // packed_bytes(...) each; work: split-K partials (ksplit*B*M'*N' floats) or null.
 void launch_gemm_packed(int dt, const GemmDesc& g, void* pa, void* pb, float* work, int ksplit,
                         hipStream_t st) {
-  OME_DRY_RETURN(ksplit > 1 ? 4 : 3);
   const bool cplx = (dt == C32);
   PackedGemmDesc d{};
   const int64_t M = g.M, N = cplx ? 2 * g.N : g.N, K = cplx ? 2 * g.K : g.K;
   int TM, TN;
   packed_tiles(M, N, TM, TN);
+  OME_DRY_RETURN((ksplit > 1 ? 2 : 1) + (TM == TN ? 1 : 2));
   // A' rows m (stride sam), k (stride sak); B' rows n (stride sbn), k (stride sbk)
-  launch_pack(g.a, (uint16_t*)pa, M, K, g.B, g.sam, g.sak, g.sab, cplx ? 1 : 0, g.M, g.K, TM, st);
-  launch_pack(g.b, (uint16_t*)pb, N, K, g.B, g.sbn, g.sbk, g.sbb, cplx ? 2 : 0, g.N, g.K, TN, st);
+  const PackDesc pka = pack_desc(g.a, (uint16_t*)pa, M, K, g.sam, g.sak, g.sab, cplx ? 1 : 0, g.M, g.K, TM);
+  const PackDesc pkb = pack_desc(g.b, (uint16_t*)pb, N, K, g.sbn, g.sbk, g.sbb, cplx ? 2 : 0, g.N, g.K, TN);
+  if (TM == TN) {
+    launch_pack2(pka, pkb, g.B, TM, st);
+  } else {
+    launch_pack(pka, g.B, TM, st);
+    launch_pack(pkb, g.B, TN, st);
+  }
   d.a = (const uint16_t*)pa;
   d.b = (const uint16_t*)pb;
   d.y = g.y;
