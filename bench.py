@@ -197,8 +197,12 @@ def bench_c4_more(ab, steps, warmup):
 
 
 def bench_c1(steps, warmup):
+    """ijk,jkl->il at 64^3 (Float64, Float32, Int64): 2*64^4 flop; bytes = both inputs and the
+    output once.  Far below both ridges (arithmetic intensity 8-16 flop/byte), so the bound is
+    launch latency."""
     import torch
     import omeinsum_mi355x as om
+    from oracle import default_backend as db
     out = {}
     code = om.ein("ijk,jkl->il")
     for name, dt in (("f64", torch.float64), ("f32", torch.float32), ("i64", torch.int64)):
@@ -211,8 +215,20 @@ def bench_c1(steps, warmup):
         sd = om.get_size_dict(code.ixs, (a, b))
         y = torch.empty(64, 64, dtype=dt, device="cuda")
         ms = time_events(lambda: om.einsum_inplace(code, (a, b), y, True, False, sd), steps * 5, warmup)
-        out[name] = {"us": ms * 1e3, "tflops": 2 * 64 ** 4 / ms / 1e9}
-    log("C1 ijk,jkl->il 64^3: %s" % {k: round(v["us"], 1) for k, v in out.items()})
+        esz = 4 if dt == torch.float32 else 8
+        nbytes = esz * (2 * 64 ** 3 + 64 * 64)
+        out[name] = {"us": ms * 1e3, "tflops": 2 * 64 ** 4 / ms / 1e9, "gbs": nbytes / ms / 1e6,
+                     "frac_hbm": nbytes / ms / 1e6 / PEAK_HBM}
+        # the CPU reference path (the oracle's DefaultBackend restatement) on the same inputs
+        an, bn = a.cpu().numpy(), b.cpu().numpy()
+        t0 = time.perf_counter()
+        reps = 0
+        while reps < 20 or time.perf_counter() - t0 < 0.2:
+            db.einsum([list("ijk"), list("jkl")], list("il"), [an, bn])
+            reps += 1
+        out[name]["cpu_us"] = (time.perf_counter() - t0) / reps * 1e6
+    log("C1 ijk,jkl->il 64^3 (GPU us / CPU us): %s" %
+        {k: (round(v["us"], 1), round(v["cpu_us"], 1)) for k, v in out.items()})
     return out
 
 
@@ -251,6 +267,10 @@ def bench_c2(steps, warmup):
                    "arena_bytes": int(info.arena_bytes), "flops": info.flops}
     v, lg = plan.execute_scalar_normalized(dxs)
     out["normalized_rel_err_fp64"] = abs(v * np.exp(lg) - ref) / abs(ref)
+    # the CPU reference path: the oracle's per-step restatement in ComplexF32
+    t0 = time.perf_counter()
+    db.nested_einsum(tree.to_plain(), xs, sd)
+    out["cpu_ms"] = (time.perf_counter() - t0) * 1e3
     om.set_einsum_backend(om.DefaultBackend())
     log("C2 100-vertex network: %s" % {k: v for k, v in out.items() if isinstance(v, dict)})
     return out
@@ -341,6 +361,28 @@ def bench_c3(steps, warmup):
             om.tensorpermute(y, x, (1, 3, 2, 0))
     out["per_problem_fp32_ms"] = time_events(per_problem, 2, 1)
     log("C3 per-problem path (one call per GEMM / permute): %.2f ms" % out["per_problem_fp32_ms"])
+    # the CPU reference path on a sample (every 16th GEMM and permute; numpy / OpenBLAS),
+    # scaled to the whole workload by flops and bytes
+    import numpy as _np
+    t_g = 0.0
+    fs = 0.0
+    for Bt, At, Ct in probs[::16]:
+        bn, an = Bt.cpu().numpy(), At.cpu().numpy()
+        t1 = time.perf_counter()
+        _np.matmul(bn, an)
+        t_g += time.perf_counter() - t1
+        fs += 2.0 * bn.shape[0] * bn.shape[1] * bn.shape[2] * an.shape[2]
+    t_p = 0.0
+    ps = 0.0
+    for x in xs[::16]:
+        xn = x.cpu().numpy()
+        t1 = time.perf_counter()
+        _np.ascontiguousarray(_np.transpose(xn, (1, 3, 2, 0)))
+        t_p += time.perf_counter() - t1
+        ps += 8.0 * xn.size
+    out["cpu_sample"] = {"gemm_tflops": fs / t_g / 1e12, "permute_gbs": ps / t_p / 1e9,
+                         "whole_step_ms_est": (flops / (fs / t_g) + perm_bytes / (ps / t_p)) * 1e3,
+                         "sample": "every 16th instance and permute, numpy/OpenBLAS"}
     del probs, xs, ys, gg, gp
     torch.cuda.empty_cache()
     return out
@@ -352,9 +394,11 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
     largest intermediate exceeds 2^sc_budget elements (2^25 ComplexF32 = 256 MiB, a plan of
     about 1 GiB) is sliced to fit (FP64 cross-slice sum).
     Inputs and plans are prepared before the timed runs (planning time reported apart).
-    Measured: the plain per-step path back to back, the request loop at saturation (every
-    request submitted at once), and the loop under Poisson arrivals at 70% of its saturation
-    rate (p50 / p99 latency from submission to completion)."""
+    Measured: the plain per-step FP32 path back to back, the request loop (BF16x3 split
+    products on the matrix cores) at saturation (every request submitted at once), and the
+    loop under Poisson arrivals at 70% of its saturation rate (p50 / p99 latency from
+    submission to completion); the loop's error against FP64 on the requests the CPU oracle
+    contracts quickly."""
     import numpy as np
     import torch
     import omeinsum_mi355x as om
@@ -402,48 +446,82 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
     log("C5 per-step path: %.1f req/s (p50 %.2f ms, p99 %.2f ms)" %
         (nreq / t_ps, out["per_step"]["p50_ms"], out["per_step"]["p99_ms"]))
 
-    with Server(hbm_budget=hbm_budget, max_active=max_active,
-                backend=om.MI355XBackend(precision="fp32")) as srv:
-        t0 = time.perf_counter()
-        plans = [srv.plan(tree, dxs, sd) for n, tree, sd, dxs, f in reqs]
-        out["device_planning_s"] = time.perf_counter() - t0
-        # warm-up pass
-        for t in [srv.submit(p, q[3]) for p, q in zip(plans, reqs)]:
-            srv.wait(t)
-        # 2. saturation: every request queued at once
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        tickets = [srv.submit(p, q[3]) for p, q in zip(plans, reqs)]
-        res = [srv.wait(t) for t in tickets]
-        t_sat = time.perf_counter() - t0
-        sat_lat = [l for _, l in res]
-        loop_vals = [scalar(y) for y, _ in res]
-        sat_rps = nreq / t_sat
-        out["loop_saturated"] = {"req_per_s": sat_rps, "tflops": flops / t_sat / 1e12,
-                                 "p50_ms": float(np.percentile(sat_lat, 50)),
-                                 "p99_ms": float(np.percentile(sat_lat, 99))}
-        # 3. Poisson arrivals at 70% of the saturated rate
-        rate = 0.7 * sat_rps
-        gaps = np.random.Generator(np.random.PCG64(5001)).exponential(1.0 / rate, nreq)
-        arrivals = np.cumsum(gaps)
-        tickets = []
-        t0 = time.perf_counter()
-        for p, q, ta in zip(plans, reqs, arrivals):
-            while time.perf_counter() - t0 < ta:
-                pass
-            tickets.append(srv.submit(p, q[3]))
-        res = [srv.wait(t) for t in tickets]
-        t_poi = time.perf_counter() - t0
-        lats = [l for _, l in res]
-        out["loop_poisson"] = {"offered_req_per_s": rate, "req_per_s": nreq / t_poi,
-                               "p50_ms": float(np.percentile(lats, 50)),
-                               "p99_ms": float(np.percentile(lats, 99))}
-    rel = [abs(a - b) / abs(b) for a, b in zip(loop_vals, per_step)]
-    out["max_rel_diff_loop_vs_per_step"] = max(rel)
-    out["speedup_req_per_s"] = sat_rps / out["per_step"]["req_per_s"]
-    log("C5 request loop: %.1f req/s saturated (%.2fx the per-step path); Poisson at %.1f req/s: "
-        "p50 %.2f ms, p99 %.2f ms" % (sat_rps, out["speedup_req_per_s"], rate,
-                                      out["loop_poisson"]["p50_ms"], out["loop_poisson"]["p99_ms"]))
+    def run_loop(prec):
+        res_out = {}
+        with Server(hbm_budget=hbm_budget, max_active=max_active,
+                    backend=om.MI355XBackend(precision=prec)) as srv:
+            t0 = time.perf_counter()
+            plans = [srv.plan(tree, dxs, sd) for n, tree, sd, dxs, f in reqs]
+            res_out["device_planning_s"] = time.perf_counter() - t0
+            # warm-up pass
+            for t in [srv.submit(p, q[3]) for p, q in zip(plans, reqs)]:
+                srv.wait(t)
+            # saturation: every request queued at once
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            tickets = [srv.submit(p, q[3]) for p, q in zip(plans, reqs)]
+            res = [srv.wait(t) for t in tickets]
+            t_sat = time.perf_counter() - t0
+            sat_lat = [l for _, l in res]
+            vals = [scalar(y) for y, _ in res]
+            sat_rps = nreq / t_sat
+            res_out["saturated"] = {"req_per_s": sat_rps, "tflops": flops / t_sat / 1e12,
+                                    "p50_ms": float(np.percentile(sat_lat, 50)),
+                                    "p99_ms": float(np.percentile(sat_lat, 99))}
+            # Poisson arrivals at 70% of the saturated rate
+            rate = 0.7 * sat_rps
+            gaps = np.random.Generator(np.random.PCG64(5001)).exponential(1.0 / rate, nreq)
+            arrivals = np.cumsum(gaps)
+            tickets = []
+            t0 = time.perf_counter()
+            for p, q, ta in zip(plans, reqs, arrivals):
+                while time.perf_counter() - t0 < ta:
+                    pass
+                tickets.append(srv.submit(p, q[3]))
+            res = [srv.wait(t) for t in tickets]
+            t_poi = time.perf_counter() - t0
+            lats = [l for _, l in res]
+            res_out["poisson"] = {"offered_req_per_s": rate, "req_per_s": nreq / t_poi,
+                                  "p50_ms": float(np.percentile(lats, 50)),
+                                  "p99_ms": float(np.percentile(lats, 99))}
+        res_out["speedup_req_per_s"] = sat_rps / out["per_step"]["req_per_s"]
+        return res_out, vals
+
+    # error against FP64 (the oracle's per-step restatement in complex128) for every request
+    # whose tree the CPU contracts in well under a second
+    from oracle import default_backend as db
+    refs = {}
+    t_cpu = 0.0
+    for q, (n, tree, sd, dxs, f) in enumerate(reqs):
+        if f <= 2e9:
+            plain = tree.eins if hasattr(tree, "slicing") else tree
+            xs64 = [x.cpu().numpy().astype(np.complex128) for x in dxs]
+            t1 = time.perf_counter()
+            refs[q] = complex(db.nested_einsum(plain.to_plain(), xs64, sd)[()])
+            t_cpu += time.perf_counter() - t1
+    # the CPU reference path on the same small requests (oracle per-step restatement, FP64)
+    out["cpu_small_requests"] = {"requests": len(refs), "req_per_s": len(refs) / t_cpu,
+                                 "sample": "requests of <= 2e9 flop, ComplexF64, one host thread "
+                                           "per GEMM call (numpy/OpenBLAS)"}
+
+    def err_stats(vals):
+        e = [abs(vals[q] - r) / abs(r) for q, r in refs.items()]
+        return {"checked": len(e), "max": max(e), "median": float(np.median(e))}
+
+    out["per_step"]["rel_err_fp64"] = err_stats(per_step)
+    for prec in ("fp32", "bf16x3"):
+        res_out, vals = run_loop(prec)
+        res_out["rel_err_fp64"] = err_stats(vals)
+        res_out["max_rel_diff_vs_per_step"] = max(abs(a - b) / abs(b) for a, b in zip(vals, per_step))
+        out["loop_" + prec] = res_out
+        log("C5 request loop (%s): %.1f req/s saturated (%.2fx the per-step path); Poisson at %.1f "
+            "req/s: p50 %.2f ms, p99 %.2f ms; error vs FP64 median %.1e max %.1e" %
+            (prec, res_out["saturated"]["req_per_s"], res_out["speedup_req_per_s"],
+             res_out["poisson"]["offered_req_per_s"], res_out["poisson"]["p50_ms"],
+             res_out["poisson"]["p99_ms"], res_out["rel_err_fp64"]["median"],
+             res_out["rel_err_fp64"]["max"]))
+    log("C5 per-step path error vs FP64: median %.1e max %.1e" %
+        (out["per_step"]["rel_err_fp64"]["median"], out["per_step"]["rel_err_fp64"]["max"]))
     return out
 
 
@@ -560,13 +638,20 @@ def main():
     }
     c5 = extra.get("c5_serving")
     if c5:
-        # the tensor-network request stream: request loop vs the plain per-step GPU path
-        line["tn_requests"] = {"req_per_s": round(c5["loop_saturated"]["req_per_s"], 1),
-                               "per_step_req_per_s": round(c5["per_step"]["req_per_s"], 1),
-                               "vs_per_step": round(c5["speedup_req_per_s"], 2),
-                               "p50_ms": round(c5["loop_poisson"]["p50_ms"], 3),
-                               "p99_ms": round(c5["loop_poisson"]["p99_ms"], 3),
-                               "offered_req_per_s": round(c5["loop_poisson"]["offered_req_per_s"], 1)}
+        # the tensor-network request stream: request loop (BF16x3 split products, and FP32)
+        # vs the plain per-step FP32 GPU path
+        def summary(r):
+            return {"req_per_s": round(r["saturated"]["req_per_s"], 1),
+                    "vs_per_step": round(r["speedup_req_per_s"], 2),
+                    "p50_ms": round(r["poisson"]["p50_ms"], 3),
+                    "p99_ms": round(r["poisson"]["p99_ms"], 3),
+                    "offered_req_per_s": round(r["poisson"]["offered_req_per_s"], 1),
+                    "rel_err_fp64_median": r["rel_err_fp64"]["median"],
+                    "rel_err_fp64_max": r["rel_err_fp64"]["max"]}
+        line["tn_requests"] = dict(summary(c5["loop_bf16x3"]),
+                                   per_step_req_per_s=round(c5["per_step"]["req_per_s"], 1),
+                                   per_step_rel_err_fp64_max=c5["per_step"]["rel_err_fp64"]["max"],
+                                   fp32_loop=summary(c5["loop_fp32"]))
     print(json.dumps(line), flush=True)
 
 
