@@ -317,7 +317,10 @@ Status run_gemm(ome_ctx* ctx, int dt, GemmDesc d, hipStream_t st, int precision)
     if (d.M == 1 && d.sak != 1) d.sam = 1;
     if (d.K == 1) { if (d.sam != 1) d.sak = 1; if (d.sbn != 1) d.sbk = 1; }
     if (d.N == 1 && d.sbk != 1) d.sbn = 1;
-    const int prec = (dt == F64) ? PREC_FP64 : (precision == PREC_BF16X3 ? PREC_BF16X3 : PREC_FP32);
+    // BF16x3 pays on the packed path (large products, above); smaller products run FP32 MFMA,
+    // as fast for them and 8 bits more precise.  A forced MFMA kernel keeps the split for tests.
+    const bool split = precision == PREC_BF16X3 && ctx->opt.gemm_kernel == GEMM_MFMA;
+    const int prec = (dt == F64) ? PREC_FP64 : (split ? PREC_BF16X3 : PREC_FP32);
     int ks = mfma_plan_ksplit(d, 0);
     Temp ws;
     if (ks > 1) {

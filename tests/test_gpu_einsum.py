@@ -297,10 +297,16 @@ def test_mfma_f32_layouts(spec, prec):
     sd = {"i": 129, "j": 300, "k": 77, "a": 65, "b": 3}
     r = rng(909)
     xs = [rand(r, [sd[l] for l in ix], np.float32) for ix in ixs]
+    from omeinsum_mi355x import _lib
+    ctx = _lib.context()
     o.set_einsum_backend(o.MI355XBackend(precision=prec))
+    # below the packed path's size BF16x3 runs as FP32 unless the MFMA kernel is forced:
+    # force it so that the in-loop split kernel is what runs here
+    ctx.set_option(_lib.OPT_GEMM_KERNEL, 2 if prec == "bf16x3" else -1)
     try:
         got = run(ixs, iy, xs, sd)
     finally:
+        ctx.set_option(_lib.OPT_GEMM_KERNEL, -1)
         o.set_einsum_backend(o.DefaultBackend())
     ref = db.einsum(ixs, iy, [x.astype(np.float64) for x in xs], sd)
     tol_k = 300
