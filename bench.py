@@ -18,6 +18,11 @@ by the MI355X backend (graph replay) next to the plain per-step GPU path; 4096 u
 GEMMs and 1024 permutes in two grouped launches next to one call per problem; the CPU
 baseline (the oracle's numpy restatement of OMEinsum's DefaultBackend on this host).
 
+roofline.sustained: the packed GEMM's inner loop run without global memory by the calibration
+program omeinsum.jl_amd/calib/mfma_loop on random operands; the matrix cores draw more power
+on random data and the part lowers its clock, so this is the rate the loop can reach on real
+data, and frac there is achieved / that rate.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|c4|c3|c5]
                        [--no-cpu] [--copy-calibration]
 """
@@ -575,6 +580,33 @@ def cpu_baseline(ab):
                       % (len(ts), med)}
 
 
+def mfma_calibration(steps=128):
+    """Sustained BF16 MFMA rate of the packed GEMM's inner loop without global memory
+    (omeinsum.jl_amd/calib/mfma_loop: same waves, fragment reads, MFMAs and barriers, LDS
+    filled once), on constant and on random operands; None when the program is not built.
+    steps=128 is the K loop of one 4096-deep product.  Runs as a child process before this
+    process touches the GPU."""
+    import subprocess
+    exe = os.path.join(ROOT, "omeinsum.jl_amd", "calib", "mfma_loop")
+    if not os.path.exists(exe):
+        log("calibration program not built (%s)" % exe)
+        return None
+    try:
+        r = subprocess.run([exe, str(steps)], capture_output=True, text=True, timeout=120)
+    except subprocess.TimeoutExpired:
+        log("calibration program timed out")
+        return None
+    if r.returncode != 0:
+        log("calibration program failed: %s" % (r.stderr or "").strip()[-500:])
+        return None
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    log("MFMA loop without global memory (%d steps): %.1f TFLOP/s bf16 on constant operands, "
+        "%.1f on random operands (%.1f FP32-equivalent)"
+        % (out["steps"], out["const_bf16_tflops"], out["random_bf16_tflops"],
+           out["random_bf16x3_tflops"]))
+    return out
+
+
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -588,6 +620,7 @@ def main():
         print("bench.py measures one GPU: this backend runs on a single MI355X (no multi-GPU "
               "path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    calib = mfma_calibration()
     import torch
     if not torch.cuda.is_available():
         print("bench.py needs the GPU", file=sys.stderr)
@@ -652,6 +685,14 @@ def main():
                                    per_step_req_per_s=round(c5["per_step"]["req_per_s"], 1),
                                    per_step_rel_err_fp64_max=c5["per_step"]["rel_err_fp64"]["max"],
                                    fp32_loop=summary(c5["loop_fp32"]))
+    if calib and kt:
+        # the clock the part sustains under this kernel's MFMA load with real (random) data
+        sus = calib["random_bf16x3_tflops"]
+        line["roofline"]["sustained"] = {
+            "tflops": sus, "frac": round(kt / sus, 4),
+            "note": "the same inner loop without global memory (calib/mfma_loop, %d steps): "
+                    "%.1f bf16 TFLOP/s on random operands, %.1f on constant operands"
+                    % (calib["steps"], calib["random_bf16_tflops"], calib["const_bf16_tflops"])}
     print(json.dumps(line), flush=True)
 
 

@@ -231,6 +231,29 @@ __device__ __forceinline__ void raw_barrier() {
 }
 constexpr int WAIT_LGKM0 = 0xC07F;  // lgkmcnt(0): this wave's LDS reads have completed
 
+// Scheduling groups for a region holding NM MFMAs, NV vector-memory loads (the LDS copies) and
+// ND LDS reads: one MFMA ahead of each load, then one ahead of each read, then the remaining
+// MFMAs (memory operations beyond NM follow in one group).  Masks: 0x008 MFMA, 0x020 VMEM
+// read, 0x100 DS read.
+template <int NM, int NV, int ND>
+__device__ __forceinline__ void sched_interleave() {
+  constexpr int PV = NV < NM ? NV : NM;
+  constexpr int PD = ND < NM - PV ? ND : NM - PV;
+#pragma unroll
+  for (int i = 0; i < PV; ++i) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < PD; ++i) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+  }
+  if constexpr (NV > PV) __builtin_amdgcn_sched_group_barrier(0x020, NV - PV, 0);
+  if constexpr (ND > PD) __builtin_amdgcn_sched_group_barrier(0x100, ND - PD, 0);
+  if constexpr (NM > PV + PD) __builtin_amdgcn_sched_group_barrier(0x008, NM - PV - PD, 0);
+}
+
 template <int TM, int TN, int STAGES>
 __global__ __launch_bounds__(256) void k_gemm_packed_bf16x3(PackedGemmDesc d) {
   constexpr int MB = TM / 64, NB = TN / 64;
@@ -415,15 +438,21 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_packed_bf16x3_pipe(Packed
   if (kt1 > d.tiles_k) kt1 = d.tiles_k;
   const uint8_t* ga = (const uint8_t*)(d.a + ((batch * d.tiles_m + tm) * d.tiles_k) * (int64_t)pk_tile_elems(TM));
   const uint8_t* gb = (const uint8_t*)(d.b + ((batch * d.tiles_n + tn) * d.tiles_k) * (int64_t)pk_tile_elems(TN));
+  // each wave copies CPW consecutive 1 KiB blocks of one operand: the first A_BYTES / 1024
+  // blocks of a stage are A's, so the source and its per-step stride are fixed per wave
+  constexpr int A_CH = A_BYTES / 1024;
+  static_assert(A_CH % CPW == 0, "no wave may straddle the A / B boundary");
+  const int c0 = w * CPW;
+  const bool wave_a = c0 < A_CH;
+  const uint8_t* gsrc = (wave_a ? ga + c0 * 1024 : gb + (c0 - A_CH) * 1024) + lane * 16;
+  const int64_t gstep = wave_a ? A_BYTES : B_BYTES;
   auto issue = [&](int64_t kt) __attribute__((always_inline)) {
-    const int sbase = (int)(kt & 1) * STAGE;
+    const int sbase = (int)(kt & 1) * STAGE + c0 * 1024;
+    const uint8_t* src = gsrc + kt * gstep;
 #pragma unroll
     for (int i = 0; i < CPW; ++i) {
-      const int c = w * CPW + i;
-      const uint8_t* src = (c * 1024 < A_BYTES) ? ga + kt * A_BYTES + c * 1024
-                                                 : gb + kt * B_BYTES + (c * 1024 - A_BYTES);
-      const int loff = __builtin_amdgcn_readfirstlane(sbase + c * 1024);
-      __builtin_amdgcn_global_load_lds((const void*)(src + lane * 16),
+      const int loff = __builtin_amdgcn_readfirstlane(sbase + i * 1024);
+      __builtin_amdgcn_global_load_lds((const void*)(src + i * 1024),
                                        (__attribute__((address_space(3))) void*)(lds + loff),
                                        16, 0, 0);
     }
@@ -484,7 +513,12 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_packed_bf16x3_pipe(Packed
   }
   // sched_barrier(0) fences the instruction scheduler so that each group of LDS reads stays
   // ahead of the MFMAs it overlaps, and the MFMAs stay ahead of the next wait.
-  for (int64_t kt = kt0; kt < kt1; ++kt) {
+  // Steady state (a copy to issue every step): after the barrier the copies of kt+2 and the
+  // reads of s0(kt+1) are interleaved one per MFMA of s1(kt), so the matrix pipe keeps running
+  // while both waves of a SIMD issue their memory operations.
+  constexpr int N_MFMA = 3 * NB * MB, N_RD = 2 * (NB + MB);
+  int64_t kt = kt0;
+  for (; kt + 2 < kt1; ++kt) {
     const int sb = (int)(kt & 1);
     // the fragments of sub-step 0 were read under the previous MFMAs; retire them here so
     // that the reads below are not waited on before the MFMAs that do not need them
@@ -497,7 +531,23 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_packed_bf16x3_pipe(Packed
     // copy of kt+1 landed (the copy of kt+2 is not issued yet), own reads of stage sb done
     __builtin_amdgcn_s_waitcnt(waitcnt_vm(0) & WAIT_LGKM0);
     raw_barrier();
-    if (kt + 2 < kt1) issue(kt + 2);
+    issue(kt + 2);
+    rd(sb ^ 1, 0, ah0, al0, bh0, bl0);
+    mm(ah1, al1, bh1, bl1);
+    sched_interleave<N_MFMA, CPW, N_RD>();
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // the last two steps: nothing left to copy
+  for (; kt < kt1; ++kt) {
+    const int sb = (int)(kt & 1);
+    __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
+    __builtin_amdgcn_sched_barrier(0);
+    rd(sb, 1, ah1, al1, bh1, bl1);
+    __builtin_amdgcn_sched_barrier(0);
+    mm(ah0, al0, bh0, bl0);
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0) & WAIT_LGKM0);
+    raw_barrier();
     if (kt + 1 < kt1) rd(sb ^ 1, 0, ah0, al0, bh0, bl0);
     __builtin_amdgcn_sched_barrier(0);
     mm(ah1, al1, bh1, bl1);

@@ -224,8 +224,20 @@ static Status do_unary(ome_ctx* ctx, int dt, const void* x, const std::vector<in
 
 // BF16x3 on packed operands: Float32 and ComplexF32 products large enough that the one-pass
 // pack (split + tile-major relayout) is amortised by the GEMM.
-static Status run_gemm_packed(ome_ctx* ctx, int dt, const GemmDesc& d, hipStream_t st) {
+static Status run_gemm_packed(ome_ctx* ctx, int dt, const GemmDesc& d0, hipStream_t st) {
   const bool cplx = dt == C32;
+  // The kernel's lanes run along m in its stores (32 consecutive m per half-wave).  When y is
+  // contiguous along n instead, compute y^T = B^T A^T (swap the operands' roles) so that every
+  // store instruction writes whole 128-byte lines rather than 64 scattered words.
+  GemmDesc d = d0;
+  if (!cplx && d.syn == 1 && d.sym != 1) {
+    std::swap(d.a, d.b);
+    std::swap(d.M, d.N);
+    std::swap(d.sam, d.sbn);
+    std::swap(d.sak, d.sbk);
+    std::swap(d.sab, d.sbb);
+    std::swap(d.sym, d.syn);
+  }
   const int64_t M = d.M, N = cplx ? 2 * d.N : d.N, K = cplx ? 2 * d.K : d.K;
   const int64_t tiles = packed_tiles_mn(M, N) * d.B;
   const int64_t ktiles = (K + 31) / 32;
