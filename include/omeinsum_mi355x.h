@@ -19,7 +19,11 @@
  *   - Element types: OME_F32/F64 (Float32/Float64), OME_C32/C64 (ComplexF32/ComplexF64,
  *     interleaved re,im), OME_I64 (Int64, wrapping two's-complement arithmetic, bit-exact),
  *     OME_TROP_F32/F64 (TropicalNumbers' Tropical{T}: a+b := max(a,b), a*b := a+b,
- *     zero = -Inf, one = 0; the stored value is the tropical number's content).
+ *     zero = -Inf, one = 0; the stored value is the tropical number's content),
+ *     OME_DD (Double64 of DoubleFloats.jl, the element type of the reference's GPU test
+ *     test/roceinsum.jl:56-72: value = hi + lo, stored as interleaved (hi, lo) Float64
+ *     pairs; double-word addition and multiplication after Joldes, Muller and Popescu,
+ *     ACM TOMS 44(2), 2017, Algorithms 6 and 12).
  *     All inputs of one call share the output's element type; the caller promotes first
  *     (OMEinsum promotes in get_output_array, src/loop_einsum.jl:48-61).
  *   - Return value: OME_OK or an ome_status code; the message is in ome_last_error(ctx).
@@ -58,7 +62,8 @@ typedef enum ome_dtype {
   OME_C64 = 3,
   OME_I64 = 4,
   OME_TROP_F32 = 5,
-  OME_TROP_F64 = 6
+  OME_TROP_F64 = 6,
+  OME_DD = 7         /* Double64 (DoubleFloats.jl): interleaved (hi, lo) Float64 pairs */
 } ome_dtype;
 
 /* Arithmetic used for Float32 / ComplexF32 products on the matrix cores. */

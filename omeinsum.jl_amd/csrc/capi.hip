@@ -439,7 +439,7 @@ int ome_permute(ome_ctx* ctx, int dtype, int ndim, const int64_t* dims, const in
 int ome_addmul(ome_ctx* ctx, int dtype, int64_t n, const void* x, void* y, const ome_scalar* sx,
                const ome_scalar* sy, int conj_x) {
   OME_CHECK_CTX(ctx);
-  if (n < 0 || !x || !y || dtype < F32 || dtype > TF64)
+  if (n < 0 || !x || !y || dtype < F32 || dtype > DT_LAST)
     return fail(ctx, OME_ERR_INVALID_ARG, "invalid addmul arguments");
   const Scal a = sx ? Scal{sx->re, sx->im, sx->i} : scal_one(dtype);
   const Scal b = sy ? Scal{sy->re, sy->im, sy->i} : scal_zero(dtype);

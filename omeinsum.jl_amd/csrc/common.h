@@ -9,13 +9,14 @@
 
 namespace ome {
 
-enum DType : int { F32 = 0, F64 = 1, C32 = 2, C64 = 3, I64 = 4, TF32 = 5, TF64 = 6 };
+enum DType : int { F32 = 0, F64 = 1, C32 = 2, C64 = 3, I64 = 4, TF32 = 5, TF64 = 6, DD = 7 };
+constexpr int DT_LAST = DD;  // element type codes run 0..DT_LAST
 
 inline size_t elem_size(int dt) {
   switch (dt) {
     case F32: case TF32: return 4;
     case F64: case I64: case TF64: case C32: return 8;
-    case C64: return 16;
+    case C64: case DD: return 16;
   }
   return 0;
 }
@@ -29,6 +30,7 @@ struct Scal {  // layout of ome_scalar
 
 struct c32 { float x, y; };
 struct c64 { double x, y; };
+struct dd { double hi, lo; };  // double-word number hi + lo, |lo| <= ulp(hi) / 2
 
 // ---- semiring arithmetic, one struct per element type ----------------------------------
 // mul/add are the semiring's product and sum; zero/one its neutral elements.
@@ -121,6 +123,59 @@ struct OpsTF64 {
   __host__ __device__ static T from(const Scal& s) { return s.re; }
   static bool is_zero(const Scal& s) { return std::isinf(s.re) && s.re < 0; }
 };
+// Double-word numbers (DoubleFloats' Double64): value = hi + lo.  The operations are the
+// published double-word algorithms of Joldes, Muller and Popescu, "Tight and rigorous error
+// bounds for basic building blocks of double-word arithmetic", ACM TOMS 44(2), 2017:
+//   add: Algorithm 6 (AccurateDWPlusDW), relative error <= 3u^2 + 13u^3;
+//   mul: Algorithm 12 (DWTimesDW3), relative error <= 4u^2;   u = 2^-53.
+// Every rounding is the one the algorithms specify: FP contraction is off in these bodies and
+// the one fused operation used is the explicit fma of the error-free product.
+struct OpsDD {
+  using T = dd;
+  static constexpr int DT = DD;
+  __host__ __device__ static T zero() { return {0.0, 0.0}; }
+  __host__ __device__ static T one() { return {1.0, 0.0}; }
+  // s + e = a + b exactly (Knuth's TwoSum)
+  __host__ __device__ static void two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
+    s = a + b;
+    const double bv = s - a;
+    e = (a - (s - bv)) + (b - bv);
+  }
+  // s + e = a + b exactly when the exponent of a is at least that of b (Dekker's Fast2Sum)
+  __host__ __device__ static void fast_two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
+    s = a + b;
+    e = b - (s - a);
+  }
+  __host__ __device__ static T add(T x, T y) {
+#pragma clang fp contract(off)
+    double sh, sl, th, tl, vh, vl;
+    two_sum(x.hi, y.hi, sh, sl);
+    two_sum(x.lo, y.lo, th, tl);
+    const double c = sl + th;
+    fast_two_sum(sh, c, vh, vl);
+    const double w = tl + vl;
+    T z;
+    fast_two_sum(vh, w, z.hi, z.lo);
+    return z;
+  }
+  __host__ __device__ static T mul(T x, T y) {
+#pragma clang fp contract(off)
+    const double ch = x.hi * y.hi;
+    const double cl1 = __builtin_fma(x.hi, y.hi, -ch);  // ch + cl1 = x.hi * y.hi exactly
+    const double tl0 = x.lo * y.lo;
+    const double tl1 = __builtin_fma(x.hi, y.lo, tl0);
+    const double cl2 = __builtin_fma(x.lo, y.hi, tl1);
+    const double cl3 = cl1 + cl2;
+    T z;
+    fast_two_sum(ch, cl3, z.hi, z.lo);
+    return z;
+  }
+  __host__ __device__ static T fma(T a, T b, T c) { return add(mul(a, b), c); }
+  __host__ __device__ static T from(const Scal& s) { return {s.re, 0.0}; }
+  static bool is_zero(const Scal& s) { return s.re == 0.0; }
+};
 
 // Dispatch a functor templated on the Ops struct by runtime dtype.
 #define OME_DISPATCH(dt, OPS, ...)                          \
@@ -132,6 +187,7 @@ struct OpsTF64 {
     case I64: { using OPS = OpsI64; __VA_ARGS__; } break;   \
     case TF32: { using OPS = OpsTF32; __VA_ARGS__; } break; \
     case TF64: { using OPS = OpsTF64; __VA_ARGS__; } break; \
+    case DD: { using OPS = OpsDD; __VA_ARGS__; } break;     \
     default: break;                                         \
   }
 

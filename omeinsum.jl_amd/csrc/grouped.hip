@@ -501,7 +501,7 @@ int ome_grouped_gemm(ome_ctx* ctx, int dtype, int ngroups, const int64_t* dims,
   if (!ctx) return OME_ERR_INVALID_ARG;
   if (ngroups < 0 || (ngroups && (!dims || !a || !b || !c)))
     return gfail(ctx, OME_ERR_INVALID_ARG, "invalid grouped GEMM arguments");
-  if (dtype < F32 || dtype > TF64) return gfail(ctx, OME_ERR_INVALID_ARG, "unknown element type");
+  if (dtype < F32 || dtype > DT_LAST) return gfail(ctx, OME_ERR_INVALID_ARG, "unknown element type");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   hipSetDevice(ctx->device);
   std::vector<GGProblem> ps(ngroups);
@@ -532,7 +532,7 @@ int ome_grouped_permute(ome_ctx* ctx, int dtype, int nperm, const int32_t* ndims
   if (!ctx) return OME_ERR_INVALID_ARG;
   if (nperm < 0 || (nperm && (!ndims || !a || !c)))
     return gfail(ctx, OME_ERR_INVALID_ARG, "invalid grouped permute arguments");
-  if (dtype < F32 || dtype > TF64) return gfail(ctx, OME_ERR_INVALID_ARG, "unknown element type");
+  if (dtype < F32 || dtype > DT_LAST) return gfail(ctx, OME_ERR_INVALID_ARG, "unknown element type");
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   hipSetDevice(ctx->device);
   const Scal vx = sx ? Scal{sx->re, sx->im, sx->i} : scal_one(dtype);

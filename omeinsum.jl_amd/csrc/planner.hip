@@ -548,7 +548,7 @@ static Status do_binary(ome_ctx* ctx, const EinsumCall& c, const std::vector<int
 Status run_einsum(ome_ctx* ctx, const EinsumCall& c0, hipStream_t st) {
   EinsumCall c = c0;
   const int dt = c.dt;
-  if (dt < F32 || dt > TF64) return Status::err(OME_ERR_INVALID_ARG, "unknown element type");
+  if (dt < F32 || dt > DT_LAST) return Status::err(OME_ERR_INVALID_ARG, "unknown element type");
   const int nin = (int)c.xs.size();
   if (nin < 1) return Status::err(OME_ERR_INVALID_ARG, "number of input tensors must be greater than 0");
   const int nl = (int)c.sizes.size();

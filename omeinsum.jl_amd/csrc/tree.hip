@@ -364,7 +364,7 @@ int ome_tree_create(ome_ctx* ctx, int dtype, const int32_t* desc, int64_t desc_l
                     const int64_t* sizes, int nslice, const int32_t* slice_labels, ome_tree** out) {
   if (!ctx || !desc || !out || desc_len < 2 || nlabels < 0 || (nlabels && !sizes))
     return tree_fail(ctx, Status::err(OME_ERR_INVALID_ARG, "invalid tree arguments"));
-  if (dtype < F32 || dtype > TF64) return tree_fail(ctx, Status::err(OME_ERR_INVALID_ARG, "unknown element type"));
+  if (dtype < F32 || dtype > DT_LAST) return tree_fail(ctx, Status::err(OME_ERR_INVALID_ARG, "unknown element type"));
   std::lock_guard<std::recursive_mutex> g(ctx->mu);
   hipSetDevice(ctx->device);
   ome_tree* t = new ome_tree();
@@ -618,7 +618,7 @@ int ome_tree_execute_scalar_normalized(ome_tree* t, const void* const* xs, doubl
   if (!t || !xs || !value || !log_scale)
     return tree_fail(t ? t->ctx : nullptr, Status::err(OME_ERR_INVALID_ARG, "null argument"));
   ome_ctx* ctx = t->ctx;
-  if (prod(t->root_iy, t->sizes) != 1 || is_tropical(t->dt) || t->dt == I64)
+  if (prod(t->root_iy, t->sizes) != 1 || is_tropical(t->dt) || t->dt == I64 || t->dt == DD)
     return tree_fail(ctx, Status::err(OME_ERR_INVALID_ARG,
                                       "normalised evaluation needs a float/complex scalar output"));
   std::lock_guard<std::recursive_mutex> g(ctx->mu);

@@ -14,7 +14,7 @@ import ctypes
 
 from . import _lib
 from .codes import EinCode, NestedEinsum, SlicedEinsum, get_size_dict
-from .device import Tropical, dtype_code, scalar, to_device
+from .device import common_kind, dtype_code, raw, scalar, to_device, wrap
 
 
 def _conj_copy(t):
@@ -40,6 +40,10 @@ def einsum_grad(ixs, xs, iy, size_dict, dy, i):
     nixs[i] = list(iy)
     nxs = [to_device(x) for x in xs]
     d = to_device(dy)
+    if common_kind(list(nxs) + [d]) is not None:
+        # wrapped element types (Tropical, Double64) are real: conj is the identity
+        nxs[i] = d
+        return flat_einsum(EinCode(nixs, list(ixs[i])), tuple(nxs), size_dict)
     nxs[i] = _conj_copy(d) if d.is_complex() else d
     y = flat_einsum(EinCode(nixs, list(ixs[i])), tuple(nxs), size_dict)
     g = _conj_copy(y) if y.is_complex() else y
@@ -69,9 +73,10 @@ def _cost_and_gradient_steps(ne, xs, ybar):
     cache = cached(ne)
     import torch
     if ybar is None:
-        if cache[0].dim() != 0:
+        if raw(cache[0]).dim() != 0:
             raise ValueError("The output must be a scalar! Or you need to feed the gradient manually.")
-        ybar = torch.ones_like(cache[0])
+        kind = common_kind([cache[0]])
+        ybar = wrap(kind, torch.full_like(raw(cache[0]), kind.ONE if kind else 1))
     grads = [None] * len(xs)
 
     def back(n, c, dy):

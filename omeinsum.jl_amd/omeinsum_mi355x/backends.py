@@ -66,15 +66,15 @@ class DefaultBackend(EinsumBackend):
         the slices column-major over the sliced labels, take_slice, evaluate, fill_slice!."""
         import itertools
         from . import ein_ops as E
-        from .device import Tropical
+        from .device import kind_of, raw, wrap
         if not se.slicing:
             return self.einsum(se.eins, xs, size_dict)
         ixsv = se.getixsv()
         iyv = se.getiyv()
         res = E._new_output(xs, [size_dict[l] for l in iyv])
-        rt = res.data if isinstance(res, Tropical) else res
-        trop = isinstance(res, Tropical)
-        rt.fill_(float("-inf") if trop else 0)
+        rt = raw(res)
+        kind = kind_of(res)
+        rt.fill_(kind.ZERO if kind else 0)
         sd = dict(size_dict)
         for l in se.slicing:
             sd[l] = 1
@@ -84,17 +84,20 @@ class DefaultBackend(EinsumBackend):
             sm = dict(zip(reversed(se.slicing), combo))
             xsi = []
             for x, ix in zip(xs, ixsv):
-                xt = x.data if isinstance(x, Tropical) else E.to_device(x)
+                xt = E.to_device(raw(x))
                 idx = tuple(sm[l] if l in sm else slice(None) for l in ix)
                 xi = xt[idx].contiguous()
-                xsi.append(Tropical(xi) if trop else xi)
+                xsi.append(wrap(kind, xi))
             resi = self.einsum(inner, tuple(xsi), sd)
-            ri = resi.data if isinstance(resi, Tropical) else resi
+            ri = raw(resi)
             idx = tuple(sm[l] if l in sm else slice(None) for l in iyv)
-            if trop:
-                rt[idx] = rt[idx].maximum(ri)
-            else:
+            if kind is None:
                 rt[idx] += ri
+            else:
+                # the semiring's own sum: y <- one*y + one*slice through the library
+                acc = rt[idx].contiguous()
+                E.tensorpermute(kind(acc), kind(ri), list(range(ri.dim())), True, True)
+                rt[idx] = acc
         return res
 
 

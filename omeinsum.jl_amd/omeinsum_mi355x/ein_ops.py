@@ -9,7 +9,7 @@ import numpy as np
 
 from . import _lib
 from .codes import EinCode, NestedEinsum, SlicedEinsum, get_size_dict
-from .device import Tropical, dtype_code, scalar, to_device
+from .device import common_kind, dtype_code, kind_of, raw, scalar, to_device, wrap
 
 
 def _torch():
@@ -18,18 +18,16 @@ def _torch():
 
 
 def _unwrap(xs):
-    """-> (device tensors, tropical flag, common torch dtype)."""
+    """-> (device tensors, wrapper class of the elements or None, common torch dtype)."""
     torch = _torch()
-    trop = [isinstance(x, Tropical) for x in xs]
-    if any(trop) and not all(trop):
-        raise TypeError("cannot mix Tropical and ordinary tensors in one contraction")
-    ts = [to_device(x.data if isinstance(x, Tropical) else x) for x in xs]
+    kind = common_kind(xs)
+    ts = [to_device(raw(x)) for x in xs]
     T = ts[0].dtype
     for t in ts[1:]:
         T = torch.promote_types(T, t.dtype)
     # promote_type of get_output_array (src/loop_einsum.jl:48-61); align_eltypes converts
     ts = [t if t.dtype == T else t.to(T) for t in ts]
-    return ts, all(trop) and len(trop) > 0, T
+    return ts, kind, T
 
 
 def _labels(ixs, iy):
@@ -64,10 +62,10 @@ def _call_flat(ixs, iy, ts, yt, code, sx, sy, size_dict, force_loop=False):
 
 
 def _flat_einsum_inplace(ixs, iy, xs, y, sx, sy, size_dict, force_loop=False):
-    ts, trop, T = _unwrap(xs)
-    yt = y.data if isinstance(y, Tropical) else y
-    if trop != isinstance(y, Tropical):
-        raise TypeError("output and inputs must both be Tropical or both ordinary")
+    ts, kind, T = _unwrap(xs)
+    yt = raw(y)
+    if kind != kind_of(y):
+        raise TypeError("output and inputs must have the same element kind")
     if yt.dtype != T:
         ts = [t.to(yt.dtype) for t in ts]
     if not yt.is_contiguous():
@@ -76,16 +74,16 @@ def _flat_einsum_inplace(ixs, iy, xs, y, sx, sy, size_dict, force_loop=False):
         if t.dim() != len(ix):
             from .codes import ArgumentError
             raise ArgumentError("indices %r invalid for tensor with ndims = %d" % (tuple(ix), t.dim()))
-    code = dtype_code(yt.dtype, trop)
+    code = dtype_code(yt.dtype, kind)
     _call_flat(ixs, iy, ts, yt, code, sx, sy, size_dict, force_loop)
     return y
 
 
 def _new_output(xs, shape):
-    ts, trop, T = _unwrap(xs)
+    ts, kind, T = _unwrap(xs)
     torch = _torch()
     y = torch.empty(tuple(int(s) for s in shape), dtype=T, device=ts[0].device)
-    return Tropical(y) if trop else y
+    return wrap(kind, y)
 
 
 def flat_einsum(code, xs, size_dict):
@@ -137,11 +135,11 @@ def allow_loops(flag):
 def tensorpermute(C, A, perm, sx=True, sy=False):
     """tensorpermute!(C, A, perm, sx, sy) (src/utils.jl:122-156) with 0-based perm:
     C = permutedims(A, perm) scaled."""
-    ts, trop, T = _unwrap([A])
+    ts, kind, T = _unwrap([A])
     a = ts[0]
-    ct = C.data if isinstance(C, Tropical) else C
+    ct = raw(C)
     ctx = _lib.context(ct.device.index)
-    code = dtype_code(ct.dtype, trop)
+    code = dtype_code(ct.dtype, kind)
     nd = a.dim()
     # row-major axis p is column-major dim nd-1-p
     dims = (ctypes.c_int64 * max(nd, 1))(*[a.shape[nd - 1 - q] for q in range(nd)])
@@ -157,11 +155,11 @@ def batched_gemm(ta, tb, alpha, A, B, beta, C):
     """_batched_gemm!(C1, C2, alpha, A, B, beta, C) (src/utils.jl:162-177) in torch's row-major
     convention: A is (batch, m, k) ('N') or (batch, k, m) ('T'), B is (batch, k, n) or
     (batch, n, k), C is (batch, m, n); C = alpha*op(A) op(B) + beta*C."""
-    ts, trop, T = _unwrap([A, B])
+    ts, kind, T = _unwrap([A, B])
     a, b = ts
-    ct = C.data if isinstance(C, Tropical) else C
+    ct = raw(C)
     ctx = _lib.context(ct.device.index)
-    code = dtype_code(ct.dtype, trop)
+    code = dtype_code(ct.dtype, kind)
     batch, m, n = ct.shape
     k = a.shape[2] if ta == "N" else a.shape[1]
     al, be = scalar(alpha, code), scalar(beta, code)
@@ -184,12 +182,12 @@ class GroupedGemm:
         self.n = len(problems)
         if not self.n:
             return
-        ts, trop, T = _unwrap([p[0] for p in problems] + [p[1] for p in problems])
+        ts, kind, T = _unwrap([p[0] for p in problems] + [p[1] for p in problems])
         na = self.n
         ct0 = problems[0][2]
-        ct0 = ct0.data if isinstance(ct0, Tropical) else ct0
+        ct0 = raw(ct0)
         self.ctx = _lib.context(ct0.device.index)
-        self.code = dtype_code(T, trop)
+        self.code = dtype_code(T, kind)
         self.dims = (ctypes.c_int64 * (4 * na))()
         self.pa = (ctypes.c_void_p * na)()
         self.pb = (ctypes.c_void_p * na)()
@@ -197,7 +195,7 @@ class GroupedGemm:
         self.keep = []
         for g, (A, B, C) in enumerate(problems):
             a, b = ts[g], ts[na + g]
-            ct = C.data if isinstance(C, Tropical) else C
+            ct = raw(C)
             batch, m, k = a.shape
             n = b.shape[2]
             if tuple(b.shape) != (batch, k, n) or tuple(ct.shape) != (batch, m, n):
@@ -236,9 +234,9 @@ class GroupedPermute:
         self.n = len(xs)
         if not self.n:
             return
-        ts, trop, T = _unwrap(list(xs))
+        ts, kind, T = _unwrap(list(xs))
         self.ctx = _lib.context(ts[0].device.index)
-        self.code = dtype_code(T, trop)
+        self.code = dtype_code(T, kind)
         n = self.n
         self.nd = (ctypes.c_int32 * n)()
         dims, cperm = [], []
@@ -246,7 +244,7 @@ class GroupedPermute:
         self.pc = (ctypes.c_void_p * n)()
         self.keep = []
         for g, (a, perm, y) in enumerate(zip(ts, perms, ys)):
-            yt = y.data if isinstance(y, Tropical) else y
+            yt = raw(y)
             r = a.dim()
             if sorted(perm) != list(range(r)):
                 raise ValueError("problem %d: %s is not a permutation of %d axes" % (g, perm, r))

@@ -15,7 +15,7 @@ import ctypes
 
 from . import _lib
 from .backends import MI355XBackend
-from .device import Tropical
+from .device import common_kind, raw
 from .tree import TreePlan, element_code
 
 
@@ -45,9 +45,9 @@ class Server:
         self.ctx.sync_stream()
         ts, ptrs = plan._ptrs(xs)
         if y is None:
-            res, yt = plan._out(ts, isinstance(xs[0], Tropical))
+            res, yt = plan._out(ts, common_kind(xs))
         else:
-            res, yt = y, (y.data if isinstance(y, Tropical) else y)
+            res, yt = y, (raw(y))
         t = ctypes.c_int64()
         _lib.check(self.ctx.lib.ome_server_submit(self.h, plan.h, ptrs,
                                                   ctypes.c_void_p(yt.data_ptr()),

@@ -9,7 +9,7 @@ import ctypes
 
 from . import _lib
 from .codes import NestedEinsum, SlicedEinsum
-from .device import Tropical, dtype_code, scalar, to_device
+from .device import common_kind, dtype_code, raw, scalar, to_device, wrap
 
 
 def tree_desc(ne, ids):
@@ -91,24 +91,24 @@ class TreePlan:
     def _ptrs(self, xs):
         ts = []
         for x in xs:
-            t = x.data if isinstance(x, Tropical) else to_device(x)
+            t = to_device(raw(x))
             ts.append(t)
         return ts, (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
 
-    def _out(self, ts, trop):
+    def _out(self, ts, kind):
         import torch
         y = torch.empty(self.out_shape, dtype=ts[0].dtype, device=ts[0].device)
-        return (Tropical(y) if trop else y), y
+        return (wrap(kind, y)), y
 
     def execute(self, xs, y=None, sx=True, sy=False):
         """einsum / einsum! of the tree on the device."""
         self.ctx.sync_stream()
-        trop = isinstance(xs[0], Tropical)
+        kind = common_kind(xs)
         ts, ptrs = self._ptrs(xs)
         if y is None:
-            res, yt = self._out(ts, trop)
+            res, yt = self._out(ts, kind)
         else:
-            res, yt = y, (y.data if isinstance(y, Tropical) else y)
+            res, yt = y, (raw(y))
         s_x, s_y = scalar(sx, self.dt), scalar(sy, self.dt)
         _lib.check(self.ctx.lib.ome_tree_execute(self.h, ptrs, ctypes.c_void_p(yt.data_ptr()),
                                                  ctypes.byref(s_x), ctypes.byref(s_y)), self.ctx.h)
@@ -130,15 +130,15 @@ class TreePlan:
         """cost_and_gradient on the device: returns (y, [xbar_i])."""
         import torch
         self.ctx.sync_stream()
-        trop = isinstance(xs[0], Tropical)
+        kind = common_kind(xs)
         ts, ptrs = self._ptrs(xs)
-        res, yt = self._out(ts, trop)
+        res, yt = self._out(ts, kind)
         if ybar is None:
             if self.out_shape != ():
                 raise ValueError("The output must be a scalar! Or you need to feed the gradient manually.")
             yb = torch.ones((), dtype=ts[0].dtype, device=ts[0].device)
         else:
-            yb = ybar.data if isinstance(ybar, Tropical) else to_device(ybar)
+            yb = to_device(raw(ybar))
         grads = [torch.empty_like(t) for t in ts]
         gp = (ctypes.c_void_p * len(grads))(*[g.data_ptr() for g in grads])
         _lib.check(self.ctx.lib.ome_tree_gradient(self.h, ptrs, ctypes.c_void_p(yb.data_ptr()),
@@ -152,15 +152,15 @@ _CACHE_MAX = 64
 
 def element_code(xs):
     """The C ABI element type of a tree's inputs (they must share one type)."""
-    trop = isinstance(xs[0], Tropical)
-    ts = [x.data if isinstance(x, Tropical) else to_device(x) for x in xs]
+    kind = common_kind(xs)
+    ts = [to_device(raw(x)) for x in xs]
     import torch
     T = ts[0].dtype
     for t in ts[1:]:
         T = torch.promote_types(T, t.dtype)
     if any(t.dtype != T for t in ts):
         raise TypeError("tree inputs must share one element type (promote first)")
-    return dtype_code(T, trop), ts
+    return dtype_code(T, kind), ts
 
 
 def plan_for(code, xs, size_dict):

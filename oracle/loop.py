@@ -14,10 +14,13 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 # element-type codes shared with the C oracle and with include/omeinsum_mi355x.h
-F32, F64, C32, C64, I64, TF32, TF64 = range(7)
+F32, F64, C32, C64, I64, TF32, TF64, DD = range(8)
 
+# numpy storage of each element type.  Double-word numbers (DoubleFloats' Double64) are held
+# as complex128 arrays whose real part is hi and imaginary part lo: the same interleaved
+# 16-byte elements as the C ABI, so numpy's element order and strides apply unchanged.
 NP_DTYPE = {F32: np.float32, F64: np.float64, C32: np.complex64, C64: np.complex128,
-            I64: np.int64, TF32: np.float32, TF64: np.float64}
+            I64: np.int64, TF32: np.float32, TF64: np.float64, DD: np.complex128}
 
 
 class _Scalar(ctypes.Structure):
@@ -46,8 +49,12 @@ def _lib():
     return _LIB
 
 
-def dtype_code(arr_dtype, tropical=False):
+def dtype_code(arr_dtype, tropical=False, dd=False):
     d = np.dtype(arr_dtype)
+    if dd:
+        if d != np.dtype(np.complex128):
+            raise TypeError("double-word arrays are stored as complex128 (hi + i*lo)")
+        return DD
     if tropical:
         return {np.dtype(np.float32): TF32, np.dtype(np.float64): TF64}[d]
     return {np.dtype(np.float32): F32, np.dtype(np.float64): F64,
@@ -64,6 +71,8 @@ def scalar(v, code):
         return s
     if code == I64:
         s.i = int(v)
+    elif code == DD:
+        s.re = float(v)
     elif code in (C32, C64):
         c = complex(v)
         s.re, s.im = c.real, c.imag
@@ -82,11 +91,13 @@ def label_ids(ixs, iy):
     return ids
 
 
-def loop_einsum(ixs, iy, xs, size_dict=None, y=None, sx=True, sy=False, tropical=False):
+def loop_einsum(ixs, iy, xs, size_dict=None, y=None, sx=True, sy=False, tropical=False,
+                dd=False):
     """loop_einsum!(ixs, iy, xs, y, sx, sy, size_dict) on numpy arrays (axis p <-> label ix[p]).
 
     All inputs are promoted to a common dtype first, as get_output_array does
-    (src/loop_einsum.jl:48-61).  Returns y (a new array unless one is given)."""
+    (src/loop_einsum.jl:48-61).  Returns y (a new array unless one is given).  dd=True: the
+    complex128 arrays hold double-word numbers (real part hi, imaginary part lo)."""
     ixs = [tuple(ix) for ix in ixs]
     iy = tuple(iy)
     if len(ixs) != len(xs):
@@ -102,7 +113,7 @@ def loop_einsum(ixs, iy, xs, size_dict=None, y=None, sx=True, sy=False, tropical
     T = np.result_type(*[np.asarray(x).dtype for x in xs])
     if y is not None:
         T = y.dtype
-    code = dtype_code(T, tropical)
+    code = dtype_code(T, tropical, dd)
     fx = [np.asarray(x, dtype=T) if np.ndim(x) == 0 else np.asfortranarray(np.asarray(x, dtype=T)) for x in xs]
     ids = label_ids(ixs, iy)
     sizes = np.zeros(max(len(ids), 1), dtype=np.int64)
@@ -129,3 +140,18 @@ def loop_einsum(ixs, iy, xs, size_dict=None, y=None, sx=True, sy=False, tropical
         y[...] = out
         return y
     return out
+
+
+def dw_op(op, x, y):
+    """The oracle's double-word + (op "+") or * (op "*") elementwise on complex128 arrays
+    (hi + i*lo) of one shape."""
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    y = np.ascontiguousarray(y, dtype=np.complex128)
+    assert x.shape == y.shape
+    z = np.empty_like(x)
+    lib = _lib()
+    lib.oracle_dw_op.restype = ctypes.c_int
+    lib.oracle_dw_op.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p]
+    lib.oracle_dw_op(0 if op == "+" else 1, x.size, x.ctypes.data, y.ctypes.data, z.ctypes.data)
+    return z

@@ -22,14 +22,20 @@
  * order is the reference's sequential order.
  *
  * Element types: 0 f32, 1 f64, 2 c32, 3 c64, 4 i64 (wrapping), 5 tropical f32,
- * 6 tropical f64 (TropicalNumbers semantics: + is max, * is +, zero -Inf, one 0).
+ * 6 tropical f64 (TropicalNumbers semantics: + is max, * is +, zero -Inf, one 0),
+ * 7 double-word (DoubleFloats' Double64, the element type of test/roceinsum.jl:56-72: an
+ * interleaved (hi, lo) pair of doubles).  DoubleFloats is a dependency of the reference's
+ * tests that is not part of the reference tree; its + and * are restated here as the
+ * published double-word algorithms of Joldes, Muller and Popescu, ACM TOMS 44(2), 2017:
+ * Algorithm 6 (AccurateDWPlusDW) and Algorithm 12 (DWTimesDW3).  They are pinned against
+ * exact rational arithmetic and those papers' error bounds in tests/test_oracle.py.
  */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
-enum { D_F32 = 0, D_F64 = 1, D_C32 = 2, D_C64 = 3, D_I64 = 4, D_TF32 = 5, D_TF64 = 6 };
+enum { D_F32 = 0, D_F64 = 1, D_C32 = 2, D_C64 = 3, D_I64 = 4, D_TF32 = 5, D_TF64 = 6, D_DD = 7 };
 
 typedef struct { double re, im; int64_t i; } oscalar;
 
@@ -45,7 +51,7 @@ static val load(int dt, const void* p, int64_t off) {
     case D_F32: case D_TF32: v.re = ((const float*)p)[off]; break;
     case D_F64: case D_TF64: v.re = ((const double*)p)[off]; break;
     case D_C32: v.re = ((const float*)p)[2 * off]; v.im = ((const float*)p)[2 * off + 1]; break;
-    case D_C64: v.re = ((const double*)p)[2 * off]; v.im = ((const double*)p)[2 * off + 1]; break;
+    case D_C64: case D_DD: v.re = ((const double*)p)[2 * off]; v.im = ((const double*)p)[2 * off + 1]; break;
     case D_I64: v.i = ((const uint64_t*)p)[off]; break;
   }
   return v;
@@ -56,7 +62,7 @@ static void store(int dt, void* p, int64_t off, val v) {
     case D_F32: case D_TF32: ((float*)p)[off] = (float)v.re; break;
     case D_F64: case D_TF64: ((double*)p)[off] = v.re; break;
     case D_C32: ((float*)p)[2 * off] = (float)v.re; ((float*)p)[2 * off + 1] = (float)v.im; break;
-    case D_C64: ((double*)p)[2 * off] = v.re; ((double*)p)[2 * off + 1] = v.im; break;
+    case D_C64: case D_DD: ((double*)p)[2 * off] = v.re; ((double*)p)[2 * off + 1] = v.im; break;
     case D_I64: ((uint64_t*)p)[off] = v.i; break;
   }
 }
@@ -73,6 +79,45 @@ static val vzero(int dt) {
   val v = {0, 0, 0};
   if (dt == D_TF32 || dt == D_TF64) v.re = -INFINITY;
   return v;
+}
+
+/* ---- double-word arithmetic (value = re + im for D_DD; |im| <= ulp(re) / 2) */
+/* s + e = a + b exactly (TwoSum) */
+static void two_sum(double a, double b, double* s, double* e) {
+  double bv;
+  *s = a + b;
+  bv = *s - a;
+  *e = (a - (*s - bv)) + (b - bv);
+}
+/* s + e = a + b exactly when exponent(a) >= exponent(b) (Fast2Sum) */
+static void fast_two_sum(double a, double b, double* s, double* e) {
+  *s = a + b;
+  *e = b - (*s - a);
+}
+/* Algorithm 6, AccurateDWPlusDW */
+static val dw_add(val x, val y) {
+  double sh, sl, th, tl, c, vh, vl, w;
+  val z = {0, 0, 0};
+  two_sum(x.re, y.re, &sh, &sl);
+  two_sum(x.im, y.im, &th, &tl);
+  c = sl + th;
+  fast_two_sum(sh, c, &vh, &vl);
+  w = tl + vl;
+  fast_two_sum(vh, w, &z.re, &z.im);
+  return z;
+}
+/* Algorithm 12, DWTimesDW3 (fma gives the exact error of the leading product) */
+static val dw_mul(val x, val y) {
+  double ch, cl1, tl0, tl1, cl2, cl3;
+  val z = {0, 0, 0};
+  ch = x.re * y.re;
+  cl1 = fma(x.re, y.re, -ch);
+  tl0 = x.im * y.im;
+  tl1 = fma(x.re, y.im, tl0);
+  cl2 = fma(x.im, y.re, tl1);
+  cl3 = cl1 + cl2;
+  fast_two_sum(ch, cl3, &z.re, &z.im);
+  return z;
 }
 
 static val vmul(int dt, val a, val b) {
@@ -94,6 +139,7 @@ static val vmul(int dt, val a, val b) {
       return r;
     case D_I64: r.i = a.i * b.i; break;
     case D_TF32: case D_TF64: r.re = a.re + b.re; break;
+    case D_DD: return dw_mul(a, b);
   }
   return round_t(dt, r);
 }
@@ -105,6 +151,7 @@ static val vadd(int dt, val a, val b) {
     case D_C32: case D_C64: r.re = a.re + b.re; r.im = a.im + b.im; break;
     case D_I64: r.i = a.i + b.i; break;
     case D_TF32: case D_TF64: r.re = a.re > b.re ? a.re : b.re; break;
+    case D_DD: return dw_add(a, b);
   }
   return round_t(dt, r);
 }
@@ -112,14 +159,14 @@ static val vadd(int dt, val a, val b) {
 static val from_scalar(int dt, const oscalar* s) {
   val v = {0, 0, 0};
   v.re = s->re; v.im = s->im; v.i = (uint64_t)s->i;
-  if (dt == D_F32 || dt == D_F64 || dt == D_TF32 || dt == D_TF64) v.im = 0;
+  if (dt == D_F32 || dt == D_F64 || dt == D_TF32 || dt == D_TF64 || dt == D_DD) v.im = 0;
   return round_t(dt, v);
 }
 
 static int is_zero(int dt, val v) {
   switch (dt) {
     case D_F32: case D_F64: return v.re == 0.0;
-    case D_C32: case D_C64: return v.re == 0.0 && v.im == 0.0;
+    case D_C32: case D_C64: case D_DD: return v.re == 0.0 && v.im == 0.0;
     case D_I64: return v.i == 0;
     default: return v.re == -INFINITY;
   }
@@ -127,7 +174,7 @@ static int is_zero(int dt, val v) {
 static int is_one(int dt, val v) {
   switch (dt) {
     case D_F32: case D_F64: return v.re == 1.0;
-    case D_C32: case D_C64: return v.re == 1.0 && v.im == 0.0;
+    case D_C32: case D_C64: case D_DD: return v.re == 1.0 && v.im == 0.0;
     case D_I64: return v.i == 1;
     default: return v.re == 0.0;
   }
@@ -221,5 +268,16 @@ int oracle_loop_einsum(int dt, int nin, const void* const* xs, const int32_t* ni
     }
   }
   free(xstride);
+  return 0;
+}
+
+/* The element arithmetic alone, for the oracle's tests: op 0 = +, 1 = *, on n pairs. */
+int oracle_dw_op(int op, int64_t n, const double* x, const double* y, double* z) {
+  for (int64_t i = 0; i < n; ++i) {
+    val a = {x[2 * i], x[2 * i + 1], 0}, b = {y[2 * i], y[2 * i + 1], 0};
+    val r = op == 0 ? dw_add(a, b) : dw_mul(a, b);
+    z[2 * i] = r.re;
+    z[2 * i + 1] = r.im;
+  }
   return 0;
 }
