@@ -23,7 +23,13 @@
  *     OME_DD (Double64 of DoubleFloats.jl, the element type of the reference's GPU test
  *     test/roceinsum.jl:56-72: value = hi + lo, stored as interleaved (hi, lo) Float64
  *     pairs; double-word addition and multiplication after Joldes, Muller and Popescu,
- *     ACM TOMS 44(2), 2017, Algorithms 6 and 12).
+ *     ACM TOMS 44(2), 2017, Algorithms 6 and 12),
+ *     OME_CTROP_F64 (TropicalNumbers' CountingTropical{Float64,Float64}: a*b := (a.n + b.n,
+ *     a.c * b.c), a+b := the larger n with its count, counts added on equal n; zero =
+ *     (-Inf, 0), one = (0, 1); a scalar's re is n and im is c),
+ *     OME_TROP_BOOL (TropicalNumbers' TropicalAndOr: + is or, * is and; a scalar's i is the
+ *     value).  The semiring types have no counterpart among the reference's GPU kernels:
+ *     OMEinsum runs them through its generic paths, which these calls replace.
  *     All inputs of one call share the output's element type; the caller promotes first
  *     (OMEinsum promotes in get_output_array, src/loop_einsum.jl:48-61).
  *   - Return value: OME_OK or an ome_status code; the message is in ome_last_error(ctx).
@@ -63,7 +69,9 @@ typedef enum ome_dtype {
   OME_I64 = 4,
   OME_TROP_F32 = 5,
   OME_TROP_F64 = 6,
-  OME_DD = 7         /* Double64 (DoubleFloats.jl): interleaved (hi, lo) Float64 pairs */
+  OME_DD = 7,        /* Double64 (DoubleFloats.jl): interleaved (hi, lo) Float64 pairs */
+  OME_CTROP_F64 = 8, /* CountingTropical{Float64,Float64}: interleaved (n, c) Float64 pairs */
+  OME_TROP_BOOL = 9  /* TropicalAndOr: one byte per element, 0 or 1 */
 } ome_dtype;
 
 /* Arithmetic used for Float32 / ComplexF32 products on the matrix cores. */

@@ -9,19 +9,24 @@
 
 namespace ome {
 
-enum DType : int { F32 = 0, F64 = 1, C32 = 2, C64 = 3, I64 = 4, TF32 = 5, TF64 = 6, DD = 7 };
-constexpr int DT_LAST = DD;  // element type codes run 0..DT_LAST
+enum DType : int {
+  F32 = 0, F64 = 1, C32 = 2, C64 = 3, I64 = 4, TF32 = 5, TF64 = 6, DD = 7, CT64 = 8, TBOOL = 9
+};
+constexpr int DT_LAST = TBOOL;  // element type codes run 0..DT_LAST
 
 inline size_t elem_size(int dt) {
   switch (dt) {
     case F32: case TF32: return 4;
     case F64: case I64: case TF64: case C32: return 8;
-    case C64: case DD: return 16;
+    case C64: case DD: case CT64: return 16;
+    case TBOOL: return 1;
   }
   return 0;
 }
 inline bool is_complex(int dt) { return dt == C32 || dt == C64; }
 inline bool is_tropical(int dt) { return dt == TF32 || dt == TF64; }
+// element types whose values are real or complex numbers (FP64 slice sums, normalisation)
+inline bool is_numeric_float(int dt) { return dt == F32 || dt == F64 || dt == C32 || dt == C64; }
 
 struct Scal {  // layout of ome_scalar
   double re, im;
@@ -31,6 +36,7 @@ struct Scal {  // layout of ome_scalar
 struct c32 { float x, y; };
 struct c64 { double x, y; };
 struct dd { double hi, lo; };  // double-word number hi + lo, |lo| <= ulp(hi) / 2
+struct ct64 { double n, c; };  // CountingTropical{Float64,Float64}: tropical value n, count c
 
 // ---- semiring arithmetic, one struct per element type ----------------------------------
 // mul/add are the semiring's product and sum; zero/one its neutral elements.
@@ -177,6 +183,38 @@ struct OpsDD {
   static bool is_zero(const Scal& s) { return s.re == 0.0; }
 };
 
+// CountingTropical{Float64,Float64} (TropicalNumbers): a (*) b = (a.n + b.n, a.c * b.c);
+// a (+) b keeps the larger n with its count and adds the counts of equal n; zero = (-Inf, 0),
+// one = (0, 1).  A scalar's re is n and im is c.
+struct OpsCT64 {
+  using T = ct64;
+  static constexpr int DT = CT64;
+  __host__ __device__ static T zero() { return {-INFINITY, 0.0}; }
+  __host__ __device__ static T one() { return {0.0, 1.0}; }
+  __host__ __device__ static T mul(T a, T b) { return {a.n + b.n, a.c * b.c}; }
+  __host__ __device__ static T add(T a, T b) {
+    if (a.n > b.n) return a;
+    if (b.n > a.n) return b;
+    return {a.n, a.c + b.c};
+  }
+  __host__ __device__ static T fma(T a, T b, T c) { return add(mul(a, b), c); }
+  __host__ __device__ static T from(const Scal& s) { return {s.re, s.im}; }
+  static bool is_zero(const Scal& s) { return std::isinf(s.re) && s.re < 0; }
+};
+// TropicalAndOr (TropicalNumbers' boolean semiring): a (+) b = a | b, a (*) b = a & b,
+// zero = false, one = true; one byte per element (0 or 1).  A scalar's i is the value.
+struct OpsTBool {
+  using T = uint8_t;
+  static constexpr int DT = TBOOL;
+  __host__ __device__ static T zero() { return 0; }
+  __host__ __device__ static T one() { return 1; }
+  __host__ __device__ static T mul(T a, T b) { return a & b; }
+  __host__ __device__ static T add(T a, T b) { return a | b; }
+  __host__ __device__ static T fma(T a, T b, T c) { return (a & b) | c; }
+  __host__ __device__ static T from(const Scal& s) { return s.i != 0 ? 1 : 0; }
+  static bool is_zero(const Scal& s) { return s.i == 0; }
+};
+
 // Dispatch a functor templated on the Ops struct by runtime dtype.
 #define OME_DISPATCH(dt, OPS, ...)                          \
   switch (dt) {                                             \
@@ -188,6 +226,8 @@ struct OpsDD {
     case TF32: { using OPS = OpsTF32; __VA_ARGS__; } break; \
     case TF64: { using OPS = OpsTF64; __VA_ARGS__; } break; \
     case DD: { using OPS = OpsDD; __VA_ARGS__; } break;     \
+    case CT64: { using OPS = OpsCT64; __VA_ARGS__; } break; \
+    case TBOOL: { using OPS = OpsTBool; __VA_ARGS__; } break; \
     default: break;                                         \
   }
 

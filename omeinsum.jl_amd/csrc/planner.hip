@@ -19,11 +19,13 @@ namespace ome {
 Scal scal_one(int dt) {
   Scal s{1.0, 0.0, 1};
   if (is_tropical(dt)) s.re = 0.0;
+  if (dt == CT64) s = Scal{0.0, 1.0, 1};
   return s;
 }
 Scal scal_zero(int dt) {
   Scal s{0.0, 0.0, 0};
   if (is_tropical(dt)) s.re = -INFINITY;
+  if (dt == CT64) s.re = -INFINITY;
   return s;
 }
 

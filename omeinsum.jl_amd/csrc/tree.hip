@@ -456,7 +456,7 @@ int ome_tree_create(ome_ctx* ctx, int dtype, const int32_t* desc, int64_t desc_l
       t->stage_off[k] = stage;
       stage += align256((int64_t)prod(drop_sliced(t->leaf_ix[k], t->slicing), t->sizes_sl) * es);
     }
-    t->acc_f64 = (dtype == F32 || dtype == F64 || dtype == C32 || dtype == C64);
+    t->acc_f64 = is_numeric_float(dtype);
     t->acc_bytes = align256(prod(t->root_iy, t->sizes) * (t->acc_f64 ? (is_complex(dtype) ? 16 : 8) : (int64_t)es));
   }
   t->stage_bytes = stage;
@@ -618,7 +618,7 @@ int ome_tree_execute_scalar_normalized(ome_tree* t, const void* const* xs, doubl
   if (!t || !xs || !value || !log_scale)
     return tree_fail(t ? t->ctx : nullptr, Status::err(OME_ERR_INVALID_ARG, "null argument"));
   ome_ctx* ctx = t->ctx;
-  if (prod(t->root_iy, t->sizes) != 1 || is_tropical(t->dt) || t->dt == I64 || t->dt == DD)
+  if (prod(t->root_iy, t->sizes) != 1 || !is_numeric_float(t->dt))
     return tree_fail(ctx, Status::err(OME_ERR_INVALID_ARG,
                                       "normalised evaluation needs a float/complex scalar output"));
   std::lock_guard<std::recursive_mutex> g(ctx->mu);

@@ -18,7 +18,8 @@ _LAZY = {
     "loop_einsum_inplace": "ein_ops", "allow_loops": "ein_ops", "tensorpermute": "ein_ops",
     "batched_gemm": "ein_ops", "grouped_gemm": "ein_ops", "grouped_permute": "ein_ops",
     "GroupedGemm": "ein_ops", "GroupedPermute": "ein_ops", "asarray": "device",
-    "asscalar": "device", "Tropical": "device", "Double64": "device", "Wrapped": "device", "to_device": "device", "to_numpy": "device",
+    "asscalar": "device", "Tropical": "device", "Double64": "device", "Wrapped": "device",
+    "CountingTropical": "device", "TropicalAndOr": "device", "to_device": "device", "to_numpy": "device",
     "EinsumBackend": "backends", "DefaultBackend": "backends", "MI355XBackend": "backends",
     "set_einsum_backend": "backends", "get_einsum_backend": "backends",
     "einsum_grad": "autodiff", "cost_and_gradient": "autodiff", "einsum_backward_rule": "autodiff",

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libomeinsum_mi355x.so")
 
 OME_OK, OME_ERR_INVALID_ARG, OME_ERR_DIM_MISMATCH, OME_ERR_OOM = 0, 1, 2, 3
 OME_ERR_LOOPS_FORBIDDEN, OME_ERR_DEVICE, OME_ERR_UNSUPPORTED = 4, 5, 6
-F32, F64, C32, C64, I64, TF32, TF64, DD = range(8)
+F32, F64, C32, C64, I64, TF32, TF64, DD, CT64, TBOOL = range(10)
 OPT_PRECISION, OPT_COMPLEX_ALGO, OPT_ALLOW_LOOPS, OPT_DIRECT_GEMM = 0, 1, 2, 3
 OPT_USE_GRAPHS, OPT_AUTOTUNE, OPT_GEMM_KERNEL, OPT_KERNEL_TIMING = 4, 5, 6, 7
 OPT_GROUPED_VARIANT, OPT_LAST_GROUPED_CHOICE, OPT_TREE_DEFER_MEMORY = 8, 9, 10

@@ -10,7 +10,10 @@ wrapped ``data`` tensor holds one element per entry, with the C ABI's element la
   * ``Tropical``: TropicalNumbers' Tropical{T} (+ is max, * is +), data float32/float64
     holding the tropical numbers' contents;
   * ``Double64``: DoubleFloats' Double64 (value hi + lo), data complex128 whose real part is
-    hi and imaginary part lo (the interleaved 16-byte pairs of the C ABI).
+    hi and imaginary part lo (the interleaved 16-byte pairs of the C ABI);
+  * ``CountingTropical``: TropicalNumbers' CountingTropical{Float64,Float64} (tropical value
+    n with a count c), data complex128 with real part n and imaginary part c;
+  * ``TropicalAndOr``: TropicalNumbers' boolean semiring (+ is or, * is and), data bool.
 """
 import numpy as np
 
@@ -118,6 +121,65 @@ class Double64(Wrapped):
         return self.data.real + self.data.imag
 
 
+class CountingTropical(Wrapped):
+    """A tensor of CountingTropical{Float64,Float64} numbers: (n, c) with
+    a * b = (a.n + b.n, a.c * b.c) and a + b = the larger n with its count (counts added on
+    equal n).  ``data`` is complex128 with real part n and imaginary part c;
+    CountingTropical(n, c) builds one from float64 tensors (c defaults to ones)."""
+
+    ZERO = complex(float("-inf"), 0.0)
+    ONE = complex(0.0, 1.0)
+
+    def __init__(self, n, c=None):
+        import torch
+        if c is None and isinstance(n, (CountingTropical, torch.Tensor)) and \
+                (isinstance(n, CountingTropical) or n.dtype == torch.complex128):
+            super().__init__(n)
+            return
+        nt = torch.as_tensor(n, dtype=torch.float64)
+        ct = torch.ones_like(nt) if c is None else torch.as_tensor(c, dtype=torch.float64,
+                                                                   device=nt.device)
+        super().__init__(torch.complex(nt, ct))
+
+    @staticmethod
+    def _coerce(data):
+        import torch
+        if data.dtype != torch.complex128:
+            raise TypeError("CountingTropical data is complex128 (real part n, imaginary part c)")
+        return data
+
+    @classmethod
+    def codes(cls):
+        import torch
+        return {torch.complex128: _lib.CT64}
+
+    @property
+    def n(self):
+        return self.data.real
+
+    @property
+    def c(self):
+        return self.data.imag
+
+
+class TropicalAndOr(Wrapped):
+    """A tensor of TropicalAndOr numbers (the boolean semiring: + is or, * is and);
+    ``data`` is a bool tensor."""
+
+    ZERO = False
+    ONE = True
+
+    @staticmethod
+    def _coerce(data):
+        import torch
+        return data if data.dtype == torch.bool else data != 0
+
+    @classmethod
+    def codes(cls):
+        import torch
+        return {torch.bool: _lib.TBOOL}
+
+
 def kind_of(x):
     """The wrapper class of x, or None for a plain tensor / array / number."""
     return type(x) if isinstance(x, Wrapped) else None
@@ -213,6 +275,16 @@ def scalar(v, code):
     """Python number / bool -> ome_scalar for element type `code`.  Bools follow Julia:
     true is the element type's one and false its zero (tropical: 0 and -Inf)."""
     s = _lib.Scalar()
+    if code == _lib.TBOOL:
+        s.i = 1 if v else 0
+        return s
+    if code == _lib.CT64:
+        if isinstance(v, (bool, np.bool_)):
+            s.re, s.im = (0.0, 1.0) if v else (float("-inf"), 0.0)
+        else:
+            c = complex(v)
+            s.re, s.im = c.real, c.imag
+        return s
     if isinstance(v, (bool, np.bool_)):
         if code in (_lib.TF32, _lib.TF64):
             s.re = 0.0 if v else -np.inf

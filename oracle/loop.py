@@ -14,13 +14,16 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 # element-type codes shared with the C oracle and with include/omeinsum_mi355x.h
-F32, F64, C32, C64, I64, TF32, TF64, DD = range(8)
+F32, F64, C32, C64, I64, TF32, TF64, DD, CT64, TBOOL = range(10)
 
 # numpy storage of each element type.  Double-word numbers (DoubleFloats' Double64) are held
 # as complex128 arrays whose real part is hi and imaginary part lo: the same interleaved
 # 16-byte elements as the C ABI, so numpy's element order and strides apply unchanged.
+# CountingTropical{Float64,Float64} is held the same way (real part n, imaginary part c), and
+# TropicalAndOr as numpy bool.
 NP_DTYPE = {F32: np.float32, F64: np.float64, C32: np.complex64, C64: np.complex128,
-            I64: np.int64, TF32: np.float32, TF64: np.float64, DD: np.complex128}
+            I64: np.int64, TF32: np.float32, TF64: np.float64, DD: np.complex128,
+            CT64: np.complex128, TBOOL: np.bool_}
 
 
 class _Scalar(ctypes.Structure):
@@ -49,8 +52,19 @@ def _lib():
     return _LIB
 
 
-def dtype_code(arr_dtype, tropical=False, dd=False):
+def dtype_code(arr_dtype, tropical=False, dd=False, semiring=None):
+    """semiring: None, "tropical", "dd", "counting" (CountingTropical) or "andor"."""
     d = np.dtype(arr_dtype)
+    if semiring == "counting":
+        if d != np.dtype(np.complex128):
+            raise TypeError("CountingTropical arrays are stored as complex128 (n + i*c)")
+        return CT64
+    if semiring == "andor":
+        if d != np.dtype(np.bool_):
+            raise TypeError("TropicalAndOr arrays are numpy bool")
+        return TBOOL
+    dd = dd or semiring == "dd"
+    tropical = tropical or semiring == "tropical"
     if dd:
         if d != np.dtype(np.complex128):
             raise TypeError("double-word arrays are stored as complex128 (hi + i*lo)")
@@ -68,6 +82,16 @@ def scalar(v, code):
     s = _Scalar()
     if isinstance(v, (bool, np.bool_)) and code in (TF32, TF64):
         s.re = 0.0 if v else -np.inf
+        return s
+    if code == CT64:
+        if isinstance(v, (bool, np.bool_)):
+            s.re, s.im = (0.0, 1.0) if v else (-np.inf, 0.0)
+        else:
+            c = complex(v)
+            s.re, s.im = c.real, c.imag
+        return s
+    if code == TBOOL:
+        s.i = 1 if v else 0
         return s
     if code == I64:
         s.i = int(v)
@@ -92,7 +116,7 @@ def label_ids(ixs, iy):
 
 
 def loop_einsum(ixs, iy, xs, size_dict=None, y=None, sx=True, sy=False, tropical=False,
-                dd=False):
+                dd=False, semiring=None):
     """loop_einsum!(ixs, iy, xs, y, sx, sy, size_dict) on numpy arrays (axis p <-> label ix[p]).
 
     All inputs are promoted to a common dtype first, as get_output_array does
@@ -113,7 +137,7 @@ def loop_einsum(ixs, iy, xs, size_dict=None, y=None, sx=True, sy=False, tropical
     T = np.result_type(*[np.asarray(x).dtype for x in xs])
     if y is not None:
         T = y.dtype
-    code = dtype_code(T, tropical, dd)
+    code = dtype_code(T, tropical, dd, semiring)
     fx = [np.asarray(x, dtype=T) if np.ndim(x) == 0 else np.asfortranarray(np.asarray(x, dtype=T)) for x in xs]
     ids = label_ids(ixs, iy)
     sizes = np.zeros(max(len(ids), 1), dtype=np.int64)

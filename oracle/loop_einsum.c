@@ -28,14 +28,18 @@
  * tests that is not part of the reference tree; its + and * are restated here as the
  * published double-word algorithms of Joldes, Muller and Popescu, ACM TOMS 44(2), 2017:
  * Algorithm 6 (AccurateDWPlusDW) and Algorithm 12 (DWTimesDW3).  They are pinned against
- * exact rational arithmetic and those papers' error bounds in tests/test_oracle.py.
+ * exact rational arithmetic and those papers' error bounds in tests/test_oracle_double64.py;
+ * 8 CountingTropical{Float64,Float64} (an interleaved (n, c) pair: * adds n and multiplies
+ * c, + keeps the larger n and adds the counts of equal n, zero (-Inf, 0), one (0, 1));
+ * 9 TropicalAndOr (one byte, + is or, * is and).
  */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
-enum { D_F32 = 0, D_F64 = 1, D_C32 = 2, D_C64 = 3, D_I64 = 4, D_TF32 = 5, D_TF64 = 6, D_DD = 7 };
+enum { D_F32 = 0, D_F64 = 1, D_C32 = 2, D_C64 = 3, D_I64 = 4, D_TF32 = 5, D_TF64 = 6, D_DD = 7,
+       D_CT64 = 8, D_BOOL = 9 };
 
 typedef struct { double re, im; int64_t i; } oscalar;
 
@@ -51,8 +55,9 @@ static val load(int dt, const void* p, int64_t off) {
     case D_F32: case D_TF32: v.re = ((const float*)p)[off]; break;
     case D_F64: case D_TF64: v.re = ((const double*)p)[off]; break;
     case D_C32: v.re = ((const float*)p)[2 * off]; v.im = ((const float*)p)[2 * off + 1]; break;
-    case D_C64: case D_DD: v.re = ((const double*)p)[2 * off]; v.im = ((const double*)p)[2 * off + 1]; break;
+    case D_C64: case D_DD: case D_CT64: v.re = ((const double*)p)[2 * off]; v.im = ((const double*)p)[2 * off + 1]; break;
     case D_I64: v.i = ((const uint64_t*)p)[off]; break;
+    case D_BOOL: v.i = ((const uint8_t*)p)[off] != 0; break;
   }
   return v;
 }
@@ -62,8 +67,9 @@ static void store(int dt, void* p, int64_t off, val v) {
     case D_F32: case D_TF32: ((float*)p)[off] = (float)v.re; break;
     case D_F64: case D_TF64: ((double*)p)[off] = v.re; break;
     case D_C32: ((float*)p)[2 * off] = (float)v.re; ((float*)p)[2 * off + 1] = (float)v.im; break;
-    case D_C64: case D_DD: ((double*)p)[2 * off] = v.re; ((double*)p)[2 * off + 1] = v.im; break;
+    case D_C64: case D_DD: case D_CT64: ((double*)p)[2 * off] = v.re; ((double*)p)[2 * off + 1] = v.im; break;
     case D_I64: ((uint64_t*)p)[off] = v.i; break;
+    case D_BOOL: ((uint8_t*)p)[off] = v.i ? 1 : 0; break;
   }
 }
 
@@ -77,7 +83,7 @@ static val round_t(int dt, val v) {
 
 static val vzero(int dt) {
   val v = {0, 0, 0};
-  if (dt == D_TF32 || dt == D_TF64) v.re = -INFINITY;
+  if (dt == D_TF32 || dt == D_TF64 || dt == D_CT64) v.re = -INFINITY;
   return v;
 }
 
@@ -140,6 +146,8 @@ static val vmul(int dt, val a, val b) {
     case D_I64: r.i = a.i * b.i; break;
     case D_TF32: case D_TF64: r.re = a.re + b.re; break;
     case D_DD: return dw_mul(a, b);
+    case D_CT64: r.re = a.re + b.re; r.im = a.im * b.im; break;
+    case D_BOOL: r.i = a.i & b.i; break;
   }
   return round_t(dt, r);
 }
@@ -152,6 +160,13 @@ static val vadd(int dt, val a, val b) {
     case D_I64: r.i = a.i + b.i; break;
     case D_TF32: case D_TF64: r.re = a.re > b.re ? a.re : b.re; break;
     case D_DD: return dw_add(a, b);
+    case D_CT64:
+      if (a.re > b.re) return a;
+      if (b.re > a.re) return b;
+      r.re = a.re;
+      r.im = a.im + b.im;
+      break;
+    case D_BOOL: r.i = a.i | b.i; break;
   }
   return round_t(dt, r);
 }
@@ -159,6 +174,7 @@ static val vadd(int dt, val a, val b) {
 static val from_scalar(int dt, const oscalar* s) {
   val v = {0, 0, 0};
   v.re = s->re; v.im = s->im; v.i = (uint64_t)s->i;
+  if (dt == D_BOOL) v.i = s->i != 0;
   if (dt == D_F32 || dt == D_F64 || dt == D_TF32 || dt == D_TF64 || dt == D_DD) v.im = 0;
   return round_t(dt, v);
 }
@@ -167,7 +183,7 @@ static int is_zero(int dt, val v) {
   switch (dt) {
     case D_F32: case D_F64: return v.re == 0.0;
     case D_C32: case D_C64: case D_DD: return v.re == 0.0 && v.im == 0.0;
-    case D_I64: return v.i == 0;
+    case D_I64: case D_BOOL: return v.i == 0;
     default: return v.re == -INFINITY;
   }
 }
@@ -175,7 +191,8 @@ static int is_one(int dt, val v) {
   switch (dt) {
     case D_F32: case D_F64: return v.re == 1.0;
     case D_C32: case D_C64: case D_DD: return v.re == 1.0 && v.im == 0.0;
-    case D_I64: return v.i == 1;
+    case D_I64: case D_BOOL: return v.i == 1;
+    case D_CT64: return v.re == 0.0 && v.im == 1.0;
     default: return v.re == 0.0;
   }
 }
