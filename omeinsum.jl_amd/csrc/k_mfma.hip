@@ -454,8 +454,17 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(GemmDesc d) {
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t batch = e / per, r = e - batch * per;
     const int64_t n = r / d.M, m = r - n * d.M;
+    // partials summed in split order; loaded 16 at a time so their latencies overlap
     T acc = O::zero();
-    for (int s = 0; s < d.ksplit; ++s) acc = O::add(acc, W[s * total + e]);
+    int s = 0;
+    for (; s + 16 <= d.ksplit; s += 16) {
+      T v[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = W[(int64_t)(s + q) * total + e];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc = O::add(acc, v[q]);
+    }
+    for (; s < d.ksplit; ++s) acc = O::add(acc, W[(int64_t)s * total + e]);
     T* yp = (T*)d.y + batch * d.syb + m * d.sym + n * d.syn;
     *yp = epilogue<O>(d.mode, sx, sy, acc, yp);
   }
@@ -467,8 +476,8 @@ int mfma_plan_ksplit(const GemmDesc& d, int requested) {
   if (requested > 0) return requested;
   const int64_t tiles = ((d.M + 127) / 128) * ((d.N + 127) / 128) * d.B;
   int ks = 1;
-  // few output tiles and a long K: split K so that ~512 workgroups run
-  while (tiles * ks < 512 && d.K / (ks * 2) >= 256) ks *= 2;
+  // few output tiles and a long K: split K (chunks of at least 64) so that ~512 workgroups run
+  while (tiles * ks < 512 && d.K / (ks * 2) >= 64) ks *= 2;
   return ks;
 }
 

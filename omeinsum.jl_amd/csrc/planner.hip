@@ -360,7 +360,7 @@ Status run_gemm(ome_ctx* ctx, int dt, GemmDesc d, hipStream_t st, int precision)
   int ks = 1;
   if (kernel == GEMM_VALU_TILE) {
     const int64_t tiles = ((d.M + 63) / 64) * ((d.N + 63) / 64) * d.B;
-    while (tiles * ks < 512 && d.K / (ks * 2) >= 128) ks *= 2;
+    while (tiles * ks < 512 && d.K / (ks * 2) >= 32) ks *= 2;
   }
   Temp ws;
   if (ks > 1) {

@@ -99,7 +99,7 @@ SIGNATURES = {
 }
 
 _lib = None
-_lock = threading.Lock()
+_lock = threading.RLock()  # context() may call load() while holding it
 
 
 def load():
@@ -156,6 +156,7 @@ class Context:
         with torch.cuda.device(device):
             check(self.lib.ome_init(device, pool_bytes, c_void_p(self.stream), ctypes.byref(h)))
         self.h = h
+        self._opts = {}  # option values set through this object (skips repeated calls)
 
     def sync_stream(self):
         """Follow torch's current stream (the pool reuses blocks in stream order)."""
@@ -166,7 +167,12 @@ class Context:
             self.stream = s
 
     def set_option(self, key, value):
-        check(self.lib.ome_set_option(self.h, key, int(value)), self.h)
+        value = int(value)
+        if self._opts.get(key) == value:
+            return
+        self._opts.pop(key, None)
+        check(self.lib.ome_set_option(self.h, key, value), self.h)
+        self._opts[key] = value
 
     def get_option(self, key):
         v = c_i64()
@@ -200,16 +206,19 @@ _contexts = {}
 
 def context(device=None):
     import torch
-    if not torch.cuda.is_available():
-        raise RuntimeError("omeinsum_mi355x needs an MI355X (gfx950) GPU; none is visible")
     if device is None:
+        if not torch.cuda.is_available():
+            raise RuntimeError("omeinsum_mi355x needs an MI355X (gfx950) GPU; none is visible")
         device = torch.cuda.current_device()
-    with _lock:
-        ctx = _contexts.get(device)
+    ctx = _contexts.get(device)
     if ctx is None:
-        ctx = Context(device)
+        if not torch.cuda.is_available():
+            raise RuntimeError("omeinsum_mi355x needs an MI355X (gfx950) GPU; none is visible")
         with _lock:
-            _contexts[device] = ctx
+            ctx = _contexts.get(device)
+            if ctx is None:
+                ctx = Context(device)
+                _contexts[device] = ctx
     ctx.sync_stream()
     return ctx
 

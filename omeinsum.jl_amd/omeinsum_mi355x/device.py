@@ -226,13 +226,37 @@ def dtype_code(t, kind=None):
         raise TypeError("element type %s is not supported by the MI355X backend" % t)
 
 
+_AS_IS_NAMES = ("float32", "float64", "complex64", "complex128", "int64", "bool")
+
+
+class _DtypeSet:
+    """torch dtypes passed through to_device unchanged (resolved on first use)."""
+
+    def __init__(self):
+        self.s = None
+
+    def __contains__(self, dt):
+        if self.s is None:
+            import torch
+            self.s = {getattr(torch, n) for n in _AS_IS_NAMES}
+        return dt in self.s
+
+
+_AS_IS = _DtypeSet()
+
+
 def to_device(x, device=None):
     """numpy array / python scalar / torch tensor -> contiguous torch tensor on the GPU."""
     torch = _torch()
     if isinstance(x, Wrapped):
         return type(x)(to_device(x.data, device))
     if device is None:
-        device = torch.device("cuda", torch.cuda.current_device())
+        cur = torch.cuda.current_device()
+        # already a contiguous tensor of a supported type on the current device: as it is
+        if isinstance(x, torch.Tensor) and x.is_cuda and x.device.index == cur and \
+                x.dtype in _AS_IS and x.is_contiguous():
+            return x
+        device = torch.device("cuda", cur)
     if isinstance(x, torch.Tensor):
         t = x
     else:

@@ -3,6 +3,7 @@ src/loop_einsum.jl:8-32, src/utils.jl:122-177) over the C ABI.
 
 Every contraction is enqueued on the GPU by libomeinsum_mi355x.so on torch's current stream.
 """
+import collections
 import ctypes
 
 import numpy as np
@@ -39,26 +40,47 @@ def _labels(ixs, iy):
     return ids
 
 
-def _call_flat(ixs, iy, ts, yt, code, sx, sy, size_dict, force_loop=False):
-    """One flat einsum on the device.  Row-major tensors are passed with reversed label
-    lists (see device.py)."""
-    ctx = _lib.context(yt.device.index)
+_ARGS = collections.OrderedDict()  # (ixs, iy, sizes) -> the ctypes label / size arrays
+_ARGS_MAX = 256
+
+
+def _flat_args(ixs, iy, size_dict):
+    """(nix, cixs, ciy, nlabels, sizes) ctypes arrays of one flat code, cached per code and
+    sizes (the arrays are read-only inputs of the C call)."""
+    key = (tuple(tuple(ix) for ix in ixs), tuple(iy))
     ids = _labels(ixs, iy)
-    sizes = (ctypes.c_int64 * max(len(ids), 1))()
-    for l, i in ids.items():
-        if l not in size_dict:
-            raise KeyError("size of label %r is unknown (pass size_info)" % (l,))
-        sizes[i] = int(size_dict[l])
+    try:
+        key = key + (tuple(int(size_dict[l]) for l in ids),)
+    except KeyError as e:
+        raise KeyError("size of label %r is unknown (pass size_info)" % (e.args[0],))
+    hit = _ARGS.get(key)
+    if hit is not None:
+        _ARGS.move_to_end(key)
+        return hit
+    sizes = (ctypes.c_int64 * max(len(ids), 1))(*key[2])
     n = len(ixs)
     nix = (ctypes.c_int32 * n)(*[len(ix) for ix in ixs])
     flat = [ids[l] for ix in ixs for l in reversed(list(ix))]
     cixs = (ctypes.c_int32 * max(len(flat), 1))(*flat)
     ciy = (ctypes.c_int32 * max(len(iy), 1))(*[ids[l] for l in reversed(list(iy))])
+    hit = (nix, cixs, ciy, len(ids), sizes)
+    _ARGS[key] = hit
+    if len(_ARGS) > _ARGS_MAX:
+        _ARGS.popitem(last=False)
+    return hit
+
+
+def _call_flat(ixs, iy, ts, yt, code, sx, sy, size_dict, force_loop=False):
+    """One flat einsum on the device.  Row-major tensors are passed with reversed label
+    lists (see device.py)."""
+    ctx = _lib.context(yt.device.index)
+    nix, cixs, ciy, nlabels, sizes = _flat_args(ixs, iy, size_dict)
+    n = len(ixs)
     ptrs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts])
     s_x, s_y = scalar(sx, code), scalar(sy, code)
     fn = ctx.lib.ome_loop_einsum if force_loop else ctx.lib.ome_einsum
     _lib.check(fn(ctx.h, code, n, ptrs, nix, cixs, ctypes.c_void_p(yt.data_ptr()), len(iy), ciy,
-                  len(ids), sizes, ctypes.byref(s_x), ctypes.byref(s_y)), ctx.h)
+                  nlabels, sizes, ctypes.byref(s_x), ctypes.byref(s_y)), ctx.h)
 
 
 def _flat_einsum_inplace(ixs, iy, xs, y, sx, sy, size_dict, force_loop=False):
