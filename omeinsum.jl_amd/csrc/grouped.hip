@@ -26,6 +26,7 @@ struct GroupEntry {
   size_t map_off = 0;
   int nprob = 0;
   int64_t total = 0;
+  int64_t split = 0;  // grouped permute: blocks of the shared-dim problems, which come first
 };
 
 inline int ilog2(double v) {
@@ -409,8 +410,75 @@ Status run_grouped_permute(ome_ctx* ctx, int dt, int np, const int32_t* ndims, c
       }
       if (cb >= 64) break;
     }
+    // shared-dim tile: a dim S that follows a whole group I of leading input dims in the input
+    // and a whole group O of leading output dims in the output; a block takes all of I and O
+    // and a chunk of S, so its reads run over pb * tsc and its writes over pa * tsc elements
+    const int64_t scap = GP_SCAP_BYTES / (int64_t)elem_size(dt);
+    int sq = -1;
+    std::vector<int> s_gi;
+    int64_t s_pa = 0, s_pb = 0, s_tsc = 0, s_score = -1;
+    for (int q = 1; q < R; ++q) {
+      std::vector<int> ci;
+      int64_t cb = 1;
+      for (int u = 0; u < R; ++u)
+        if (os[u] < os[q]) { ci.push_back(u); cb *= od[u]; }
+      if (ci.empty() || cb != os[q]) continue;  // S must follow I contiguously in the input
+      bool overlap = false;
+      for (int u : ci)
+        if (u < q) overlap = true;               // I and O = output dims 0..q-1 disjoint
+      const int64_t ca = ost[q];
+      if (overlap || q > 4 || (int)ci.size() > 4 || ca > 256 || cb > 256 || ca * cb * 2 > scap)
+        continue;
+      std::sort(ci.begin(), ci.end(), [&](int u, int v) { return os[u] < os[v]; });
+      const int64_t tsc = std::min<int64_t>(od[q], scap / (ca * cb));
+      const int64_t score = std::min(ca, cb) * tsc;
+      if (score > s_score) {
+        s_score = score;
+        sq = q;
+        s_gi = ci;
+        s_pa = ca;
+        s_pb = cb;
+        s_tsc = tsc;
+      }
+    }
     int64_t blocks;
-    if (!gi.empty() && !go.empty() && os[gi[0]] == 1 && R > 1) {
+    if (sq >= 0 && s_score > best_score) {
+      pr.tiled = 2;
+      pr.no = sq;
+      pr.ni = (int32_t)s_gi.size();
+      pr.pa = s_pa;
+      pr.pb = s_pb;
+      pr.tla = (int32_t)s_pa;
+      pr.tlb = (int32_t)s_pb;
+      pr.tla_div = make_fdiv((uint32_t)s_pa);
+      pr.tlb_div = make_fdiv((uint32_t)s_pb);
+      pr.tsc = (int32_t)s_tsc;
+      pr.tsc_div = make_fdiv((uint32_t)s_tsc);
+      pr.s_ext = od[sq];
+      pr.s_is = os[sq];
+      pr.s_os = ost[sq];
+      pr.ts_tiles = (od[sq] + s_tsc - 1) / s_tsc;
+      for (int u = 0; u < sq; ++u) {
+        pr.odv[u] = make_fdiv((uint32_t)od[u]);
+        pr.o_is[u] = os[u];
+      }
+      for (int u = 0; u < pr.ni; ++u) {
+        pr.idv[u] = make_fdiv((uint32_t)od[s_gi[u]]);
+        pr.i_os[u] = ost[s_gi[u]];
+      }
+      int nr = 0;
+      int64_t restn = 1;
+      for (int q = 0; q < R; ++q) {
+        if (q <= sq || std::find(s_gi.begin(), s_gi.end(), q) != s_gi.end()) continue;
+        pr.rdv[nr] = make_fdiv((uint32_t)od[q]);
+        pr.r_is[nr] = os[q];
+        pr.r_os[nr] = ost[q];
+        restn *= od[q];
+        ++nr;
+      }
+      pr.nr = nr;
+      blocks = pr.ts_tiles * restn;
+    } else if (!gi.empty() && !go.empty() && os[gi[0]] == 1 && R > 1) {
       pr.tiled = 1;
       pr.no = (int32_t)go.size();
       pr.ni = (int32_t)gi.size();
@@ -451,10 +519,29 @@ Status run_grouped_permute(ome_ctx* ctx, int dt, int np, const int32_t* ndims, c
       blocks = (n + 1023) / 1024;
     }
     probs.push_back(pr);
-    first.push_back(total);
-    total += blocks;
+    first.push_back(blocks);  // block count for now; turned into first blocks below
   }
   if (probs.empty()) return Status::ok();
+  // shared-dim problems first (a kernel instantiation of their own), then the others, each
+  // in call order
+  std::stable_partition(probs.begin(), probs.end(),
+                        [](const GPProb& q) { return q.tiled == 2; });
+  // the partition moved the problems: recompute each one's block count in the new order
+  for (size_t q = 0; q < probs.size(); ++q) {
+    const GPProb& pr = probs[q];
+    int64_t restn = 1;
+    for (int u = 0; u < pr.nr; ++u) restn *= pr.rdv[u].d;
+    first[q] = pr.tiled == 2 ? pr.ts_tiles * restn
+             : pr.tiled == 1 ? pr.ta * pr.tb * restn
+                             : (pr.n + 1023) / 1024;
+  }
+  int64_t split = 0;
+  for (size_t q = 0; q < probs.size(); ++q) {
+    const int64_t nb = first[q];
+    first[q] = total;
+    if (probs[q].tiled == 2) split += nb;
+    total += nb;
+  }
   const size_t pbytes = (probs.size() * sizeof(GPProb) + 255) & ~(size_t)255;
   const size_t fbytes = (first.size() * sizeof(int64_t) + 255) & ~(size_t)255;
   const size_t bytes = pbytes + fbytes + total * sizeof(int32_t);
@@ -470,6 +557,7 @@ Status run_grouped_permute(ome_ctx* ctx, int dt, int np, const int32_t* ndims, c
     en.map_off = pbytes + fbytes;
     en.nprob = (int)probs.size();
     en.total = total;
+    en.split = split;
   });
   if (!e) return Status::err(OME_ERR_OOM, "no memory for the grouped permute descriptors");
   GPDesc g{};
@@ -480,7 +568,7 @@ Status run_grouped_permute(ome_ctx* ctx, int dt, int np, const int32_t* ndims, c
   g.total_blocks = e->total;
   g.sx = sx;
   g.sy = sy;
-  launch_grouped_permute(dt, g, (int)std::min<int64_t>(e->total, 1 << 30), st);
+  launch_grouped_permute(dt, g, e->split, st);
   return Status::ok();
 }
 

@@ -584,19 +584,112 @@ void launch_grouped_gemm(int dt, int variant, const GGDesc& g, int grid, hipStre
 
 // ---- grouped permute ----------------------------------------------------------------------
 
-template <class O>
+// SHARED: the instantiation for shared-dim problems (tiled == 2, launched over their block
+// range); otherwise the one for the two-group tile (tiled == 1) and the plain walk.  Each
+// branch gets its own register allocation and LDS footprint.
+template <class O, bool SHARED>
 __global__ __launch_bounds__(256) void k_grouped_permute(GPDesc g) {
   using T = typename O::T;
-  __shared__ T tile[4096 + 256];  // tla rows of tlb + 1 (tla <= 256)
-  __shared__ int64_t soff_a[256], soff_b[64];
+  // two-group tile: tla rows of tlb + 1 (tla <= 256, tla * tlb <= 4096); shared-dim tile: tla
+  // rows of LDA = tsc * tlb + (1 or 2) with tla * tlb * tsc <= GP_SCAP_BYTES / sizeof(T)
+  constexpr int SCAP = GP_SCAP_BYTES / (int)sizeof(T);
+  constexpr int TILE = SHARED ? SCAP + 512 : 4096 + 256;
+  __shared__ T tile[TILE];
+  __shared__ int64_t soff_a[256], soff_b[SHARED ? 256 : 64];
   const T sx = O::from(g.sx), sy = O::from(g.sy);
-  for (int64_t blk = blockIdx.x; blk < g.total_blocks; blk += gridDim.x) {
+  for (int64_t blk = g.blk_begin + blockIdx.x; blk < g.blk_end; blk += gridDim.x) {
     const int p = g.map[blk];
     const GPProb& pr = g.probs[p];
     const T* __restrict__ x = (const T*)pr.x;
     T* __restrict__ y = (T*)pr.y;
     int64_t b = blk - g.first[p];
-    if (pr.tiled) {
+    if constexpr (SHARED) {
+      const uint32_t sk = (uint32_t)(b % pr.ts_tiles);
+      uint32_t rest = (uint32_t)(b / pr.ts_tiles);
+      int64_t ibase = 0, obase = 0;
+      for (int q = 0; q < pr.nr; ++q) {
+        const uint32_t qq = fdiv_q(rest, pr.rdv[q]);
+        const uint32_t v = rest - qq * pr.rdv[q].d;
+        rest = qq;
+        ibase += (int64_t)v * pr.r_is[q];
+        obase += (int64_t)v * pr.r_os[q];
+      }
+      const int tla = (int)pr.pa, tlb = (int)pr.pb, tsc = pr.tsc;
+      const int64_t s0 = (int64_t)sk * tsc;
+      const uint32_t nsv = (uint32_t)((pr.s_ext - s0 < tsc) ? pr.s_ext - s0 : tsc);
+      ibase += s0 * pr.s_is;
+      obase += s0 * pr.s_os;
+      // input offset of (a, s, b) = b + s * s_is + fa(a); output offset = a + s * s_os + fb(b)
+      for (int u = threadIdx.x; u < tla; u += 256) {
+        uint32_t ra = (uint32_t)u;
+        int64_t off = 0;
+        for (int q = 0; q < pr.no; ++q) {
+          const uint32_t qq = fdiv_q(ra, pr.odv[q]);
+          off += (int64_t)(ra - qq * pr.odv[q].d) * pr.o_is[q];
+          ra = qq;
+        }
+        soff_a[u] = off;
+      }
+      for (int u = threadIdx.x; u < tlb; u += 256) {
+        uint32_t rb = (uint32_t)u;
+        int64_t off = 0;
+        for (int q = 0; q < pr.ni; ++q) {
+          const uint32_t qq = fdiv_q(rb, pr.idv[q]);
+          off += (int64_t)(rb - qq * pr.idv[q].d) * pr.i_os[q];
+          rb = qq;
+        }
+        soff_b[u] = off;
+      }
+      __syncthreads();
+      const uint32_t st = (uint32_t)(tsc * tlb);
+      const uint32_t lda = st + 1 + ((st + 1) % 2 == 0 ? 1 : 0);  // odd row pitch
+      const uint32_t cnt = (uint32_t)tla * st;                     // <= SCAP
+      const T* __restrict__ xb = x + ibase;
+      T* __restrict__ yb = y + obase;
+      // read: (b, s) runs contiguous in x across lanes, a slowest
+      constexpr int NBT = sizeof(T) >= 16 ? 8 : 16;
+      for (uint32_t h = 0; h < cnt; h += 256u * NBT) {
+        T v[NBT];
+        uint32_t li[NBT];
+#pragma unroll
+        for (int j = 0; j < NBT; ++j) {
+          const uint32_t e = h + threadIdx.x + 256u * j;
+          const uint32_t t1 = fdiv_q(e, pr.tlb_div), bl = e - t1 * (uint32_t)tlb;
+          const uint32_t al = fdiv_q(t1, pr.tsc_div), sl = t1 - al * (uint32_t)tsc;
+          li[j] = (e < cnt) ? al * lda + sl * (uint32_t)tlb + bl : 0xFFFFFFFFu;
+          v[j] = O::zero();
+          if (e < cnt && sl < nsv) v[j] = xb[bl + (int64_t)sl * pr.s_is + soff_a[al]];
+        }
+#pragma unroll
+        for (int j = 0; j < NBT; ++j)
+          if (li[j] != 0xFFFFFFFFu) tile[li[j]] = v[j];
+      }
+      __syncthreads();
+      // write: (a, s) runs contiguous in y across lanes, b slowest; 8 elements per thread
+      // gathered from LDS before their stores
+      for (uint32_t h = 0; h < cnt; h += 256u * 8) {
+        T v[8];
+        int64_t yo[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const uint32_t e = h + threadIdx.x + 256u * j;
+          const uint32_t t1 = fdiv_q(e, pr.tla_div), al = e - t1 * (uint32_t)tla;
+          const uint32_t bl = fdiv_q(t1, pr.tsc_div), sl = t1 - bl * (uint32_t)tsc;
+          yo[j] = -1;
+          if (e < cnt && sl < nsv) {
+            v[j] = tile[al * lda + sl * (uint32_t)tlb + bl];
+            yo[j] = al + (int64_t)sl * pr.s_os + soff_b[bl];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (yo[j] >= 0) {
+            T* yp = yb + yo[j];
+            *yp = epilogue<O>(g.mode, sx, sy, v[j], yp);
+          }
+      }
+      __syncthreads();
+    } else if (pr.tiled) {
       const int64_t tbk = b % pr.tb;  // b-tiles fastest: consecutive blocks read neighbouring x
       b /= pr.tb;
       const int64_t tak = b % pr.ta;
@@ -696,14 +789,26 @@ __global__ __launch_bounds__(256) void k_grouped_permute(GPDesc g) {
   }
 }
 
-void launch_grouped_permute(int dt, const GPDesc& g, int grid, hipStream_t st) {
+void launch_grouped_permute(int dt, const GPDesc& g, int64_t split, hipStream_t st) {
   if (g.total_blocks <= 0) return;
-  OME_DRY_RETURN(1);
+  OME_DRY_RETURN((split > 0 ? 1 : 0) + (g.total_blocks > split ? 1 : 0));
   TimedScope ts(KC_PERMUTE, st);
   OME_DISPATCH(dt, O, {
     GPDesc gg = g;
     gg.mode = epi_mode<O>(g.sx, g.sy);
-    hipLaunchKernelGGL(k_grouped_permute<O>, dim3(grid), dim3(256), 0, st, gg);
+    if (split > 0) {
+      gg.blk_begin = 0;
+      gg.blk_end = split;
+      hipLaunchKernelGGL((k_grouped_permute<O, true>), dim3((uint32_t)std::min<int64_t>(split, 1 << 30)),
+                         dim3(256), 0, st, gg);
+    }
+    if (g.total_blocks > split) {
+      gg.blk_begin = split;
+      gg.blk_end = g.total_blocks;
+      hipLaunchKernelGGL((k_grouped_permute<O, false>),
+                         dim3((uint32_t)std::min<int64_t>(g.total_blocks - split, 1 << 30)), dim3(256), 0,
+                         st, gg);
+    }
   });
 }
 

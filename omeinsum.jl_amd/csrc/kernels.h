@@ -101,6 +101,7 @@ void gg_tile_shape(int dt, int variant, int& tm, int& tn);
 void launch_grouped_gemm(int dt, int variant, const GGDesc& g, int grid, hipStream_t st);
 
 constexpr int GP_MAXR = 8;
+constexpr int GP_SCAP_BYTES = 32768;  // LDS bytes of a shared-dim permute tile
 // One permute.  Plain path (tiled == 0): output dims (fastest first) with their input strides.
 // Tiled path: the leading output dims (group O, linear index a, contiguous in y) and the
 // leading input dims (group I, linear index b, contiguous in x) are disjoint; a block moves
@@ -125,6 +126,11 @@ struct GPProb {
   int64_t i_os[4];
   FDiv rdv[GP_MAXR];         // R dims, input and output strides
   int64_t r_is[GP_MAXR], r_os[GP_MAXR];
+  // tiled == 2: the whole of I and O and a chunk of tsc values of the shared dim S, the dim
+  // that follows I in the input and O in the output (runs of pb * tsc and pa * tsc elements)
+  int32_t tsc;
+  FDiv tsc_div;
+  int64_t s_ext, s_is, s_os, ts_tiles;
 };
 struct GPDesc {
   const GPProb* probs;
@@ -134,8 +140,10 @@ struct GPDesc {
   int64_t total_blocks;
   Scal sx, sy;
   int mode;
+  int64_t blk_begin, blk_end;  // the block range of one launch
 };
-void launch_grouped_permute(int dt, const GPDesc& g, int grid, hipStream_t st);
+// blocks [0, split) are the shared-dim problems (tiled == 2), [split, total) the others
+void launch_grouped_permute(int dt, const GPDesc& g, int64_t split, hipStream_t st);
 
 int mfma_plan_ksplit(const GemmDesc& d, int requested);
 int64_t mfma_workspace(int dt, const GemmDesc& d, int ksplit);

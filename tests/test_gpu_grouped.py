@@ -171,3 +171,48 @@ def test_grouped_permute(dtype):
     torch.cuda.synchronize()
     for y, ref, p in zip(ys, refs, prev):
         assert np.array_equal(o.to_numpy(y), 2 * ref - p)
+
+
+@pytest.mark.parametrize("kind", ["f32", "f64", "c128", "bool"])
+def test_grouped_permute_shared_dim_tiles(kind):
+    # permutes whose input-leading and output-leading dims are followed by the same dim take
+    # the shared-dim tile (a whole input group x a whole output group x a chunk of that dim);
+    # extents chosen so that the last chunk is partial and so that 1-, 4-, 8- and 16-byte
+    # elements all tile (bool through TropicalAndOr)
+    o = om()
+    r = rng(3150)
+    # row-major shapes and torch perms; (1, 3, 2, 0) on (l, k, j, i) is the column-major
+    # ijkl -> ljik of the uneven workload, where S = j follows I = i and O = l
+    cases = [((3, 300, 7, 5), (1, 3, 2, 0)),
+             ((4, 4, 4, 4), (1, 3, 2, 0)),
+             ((60, 17, 33, 64), (1, 3, 2, 0)),
+             ((9, 1000, 11), (2, 1, 0)),          # reversal: S = the middle dim
+             ((2, 3, 5, 7, 11), (4, 2, 1, 3, 0)),
+             ((200, 70, 3), (2, 1, 0)),
+             ((5, 7, 30, 3), (3, 1, 0, 2))]       # no shared dim: the two-group tile
+    xs, perms, ys, refs = [], [], [], []
+    for shape, perm in cases:
+        if kind == "bool":
+            x = r.random(shape) < 0.5
+        elif kind == "c128":
+            x = (r.integers(-8, 9, shape) + 1j * r.integers(-8, 9, shape)).astype(np.complex128)
+        else:
+            x = r.integers(-8, 9, shape).astype(np.float32 if kind == "f32" else np.float64)
+        t = torch.from_numpy(x).cuda()
+        y = torch.empty([shape[p] for p in perm], dtype=t.dtype, device="cuda")
+        if kind == "bool":
+            t, y = o.TropicalAndOr(t), o.TropicalAndOr(y)
+        xs.append(t)
+        perms.append(list(perm))
+        ys.append(y)
+        refs.append(np.transpose(x, perm))
+    o.grouped_permute(xs, perms, ys)
+    torch.cuda.synchronize()
+    for y, ref, (shape, perm) in zip(ys, refs, cases):
+        assert np.array_equal(o.to_numpy(y), ref), (shape, perm)
+    if kind != "bool":
+        prev = [o.to_numpy(y).copy() for y in ys]
+        o.grouped_permute(xs, perms, ys, 2, -1)
+        torch.cuda.synchronize()
+        for y, ref, p in zip(ys, refs, prev):
+            assert np.array_equal(o.to_numpy(y), 2 * ref - p)
