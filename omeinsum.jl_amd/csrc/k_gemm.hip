@@ -120,6 +120,46 @@ __global__ __launch_bounds__(256) void k_gemm_splitk_reduce(GemmDesc d) {
   }
 }
 
+template <class O>
+__global__ __launch_bounds__(256) void k_gemm_strided(StridedGemmDesc d) {
+  using T = typename O::T;
+  __shared__ int32_t ska[SG_KCAP], skb[SG_KCAP];
+  for (int k = threadIdx.x; k < d.nk; k += 256) {
+    uint32_t r = (uint32_t)k;
+    int32_t oa = 0, ob = 0;
+    for (int q = 0; q < d.ni; ++q) {
+      const uint32_t qq = fdiv_q(r, d.kdiv[q]);
+      const int32_t v = (int32_t)(r - qq * d.kdiv[q].d);
+      oa += v * d.ka[q];
+      ob += v * d.kb[q];
+      r = qq;
+    }
+    ska[k] = oa;
+    skb[k] = ob;
+  }
+  __syncthreads();
+  const T* __restrict__ A = (const T*)d.a;
+  const T* __restrict__ B = (const T*)d.b;
+  T* __restrict__ Y = (T*)d.y;
+  const T sx = O::from(d.sx), sy = O::from(d.sy);
+  for (int64_t o = blockIdx.x * 256ll + threadIdx.x; o < d.nout; o += (int64_t)gridDim.x * 256) {
+    uint32_t r = (uint32_t)o;
+    int32_t pa = 0, pb = 0, py = 0;
+    for (int q = 0; q < d.no; ++q) {
+      const uint32_t qq = fdiv_q(r, d.odiv[q]);
+      const int32_t v = (int32_t)(r - qq * d.odiv[q].d);
+      pa += v * d.oa[q];
+      pb += v * d.ob[q];
+      py += v * d.oy[q];
+      r = qq;
+    }
+    T acc = O::zero();
+    for (int k = 0; k < d.nk; ++k) acc = O::fma(A[pa + ska[k]], B[pb + skb[k]], acc);
+    T* yp = Y + py;
+    *yp = epilogue<O>(d.mode, sx, sy, acc, yp);
+  }
+}
+
 // One thread per output element, K walked directly from global memory: for tiny M/N or K
 // (tensor-network steps over dimension-2 labels) where a 64x64 tile would idle.
 template <class O>
@@ -244,6 +284,19 @@ void launch_gemm_valu(int dt, GemmDesc d, int kernel, int ksplit, hipStream_t st
       if (ksplit > 1)
         hipLaunchKernelGGL(k_gemm_splitk_reduce<O>, dim3(grid_cap(outputs)), dim3(256), 0, st, d);
     }
+  });
+}
+
+void launch_gemm_strided(int dt, const StridedGemmDesc& d, hipStream_t st) {
+  if (d.nout <= 0) return;
+  OME_DRY_RETURN(1);
+  TimedScope ts(KC_GEMM_VALU, st);
+  int64_t g = (d.nout + 255) / 256;
+  if (g > 4096) g = 4096;
+  OME_DISPATCH(dt, O, {
+    StridedGemmDesc dd = d;
+    dd.mode = epi_mode<O>(d.sx, d.sy);
+    hipLaunchKernelGGL(k_gemm_strided<O>, dim3((uint32_t)g), dim3(256), 0, st, dd);
   });
 }
 

@@ -56,6 +56,28 @@ void launch_walk(int dt, const WalkDesc& d, hipStream_t st);
 int64_t gemm_valu_workspace(int dt, const GemmDesc& d, int ksplit);
 void launch_gemm_valu(int dt, GemmDesc d, int kernel, int ksplit, hipStream_t st);
 
+// Binary contraction with a short contracted space read in place through per-label strides:
+// y[o] = sum_k A[fa(o) + ka(k)] * B[fb(o) + kb(k)], one output per thread, the contracted
+// offsets tabulated in LDS per block.  Replaces permute + GEMM (+ permute) for the many small
+// steps of tensor-network trees, whose labels are scattered over the operands.
+constexpr int SG_MAXO = 40;     // output labels
+constexpr int SG_MAXK = 16;     // contracted labels
+constexpr int SG_KCAP = 1024;   // contracted points
+struct StridedGemmDesc {
+  const void* a;
+  const void* b;
+  void* y;
+  int64_t nout;
+  int32_t nk, no, ni;
+  FDiv odiv[SG_MAXO];  // output labels in y's order (first fastest)
+  int32_t oa[SG_MAXO], ob[SG_MAXO], oy[SG_MAXO];
+  FDiv kdiv[SG_MAXK];  // contracted labels (first fastest)
+  int32_t ka[SG_MAXK], kb[SG_MAXK];
+  Scal sx, sy;
+  int mode;
+};
+void launch_gemm_strided(int dt, const StridedGemmDesc& d, hipStream_t st);
+
 // Few outputs (M*N*batch <= 1024), long K: one workgroup per output and K part.
 int dot_plan_parts(const GemmDesc& d);
 void launch_gemm_dot(int dt, GemmDesc d, int parts, hipStream_t st);

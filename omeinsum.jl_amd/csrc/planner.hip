@@ -405,6 +405,11 @@ static std::vector<int> order_in(const std::vector<int>& grp, const std::vector<
   return out;
 }
 
+// a binary step goes through the strided kernel when its contracted space has at most this
+// many points and output points x contracted points stays below the work bound
+constexpr int64_t STRIDED_MAX_K = 64;
+constexpr int64_t STRIDED_MAX_WORK = 1 << 24;
+
 static Status do_binary(ome_ctx* ctx, const EinsumCall& c, const std::vector<int>& ix1,
                         const std::vector<int>& ix2, const std::vector<int>& iy, const void* x1,
                         const void* x2, hipStream_t st, std::vector<Temp>& keep) {
@@ -478,6 +483,49 @@ static Status do_binary(ome_ctx* ctx, const EinsumCall& c, const std::vector<int
             best.y_inplace = yi;
           }
         }
+  // Small steps of tensor-network trees (a short contracted space, labels scattered over the
+  // operands): one strided kernel instead of permute + GEMM (+ permute).
+  {
+    const int64_t nk = prod_sizes(in, sz);
+    const bool all_inplace = best.a_inplace && best.b_inplace && best.y_inplace;
+    if (ctx->opt.gemm_kernel == GEMM_AUTO && !all_inplace && !a_pre && !b_pre && !y_post &&
+        nk <= STRIDED_MAX_K && nyy * nk <= STRIDED_MAX_WORK && (int)in.size() <= SG_MAXK &&
+        (int)iy.size() <= SG_MAXO && na < (1ll << 31) && nb < (1ll << 31) && nyy < (1ll << 31)) {
+      auto stride_in = [&](const std::vector<int>& ls, int l) -> int32_t {
+        int64_t s = 1;
+        for (int l2 : ls) {
+          if (l2 == l) return (int32_t)s;
+          s *= sz[l2];
+        }
+        return 0;
+      };
+      StridedGemmDesc d{};
+      d.a = x1;
+      d.b = x2;
+      d.y = c.y;
+      d.nout = nyy;
+      d.nk = (int32_t)nk;
+      d.no = (int32_t)iy.size();
+      d.ni = (int32_t)in.size();
+      for (int q = 0; q < d.no; ++q) {
+        const int l = iy[q];
+        d.odiv[q] = make_fdiv((uint32_t)sz[l]);
+        d.oa[q] = stride_in(ix1, l);
+        d.ob[q] = stride_in(ix2, l);
+        d.oy[q] = stride_in(iy, l);
+      }
+      for (int q = 0; q < d.ni; ++q) {
+        const int l = in[q];
+        d.kdiv[q] = make_fdiv((uint32_t)sz[l]);
+        d.ka[q] = stride_in(ix1, l);
+        d.kb[q] = stride_in(ix2, l);
+      }
+      d.sx = c.sx;
+      d.sy = c.sy;
+      launch_gemm_strided(dt, d, st);
+      return Status::ok();
+    }
+  }
   const BinLayout& L = best.L;
   auto cat = [](std::initializer_list<const std::vector<int>*> parts) {
     std::vector<int> v;
