@@ -308,6 +308,173 @@ Status run_grouped_gemm(ome_ctx* ctx, int dt, const std::vector<GGProblem>& ps, 
 
 // ---- grouped permute ------------------------------------------------------------------------
 
+// Tile plan of one permute whose output dims od (fastest first, size-1 dims dropped, runs of
+// consecutive input dims merged) have input strides os: fills pr (x, y, n, the plain walk and,
+// when one applies, the two-group or the shared-dim tile) and returns its block count.
+static int64_t gp_plan(int dt, const std::vector<int64_t>& od, const std::vector<int64_t>& os,
+                       const void* x, void* y, int64_t n, GPProb& pr) {
+  pr = GPProb{};
+  pr.x = x;
+  pr.y = y;
+  pr.n = n;
+  pr.rank = (int32_t)od.size();
+  for (int q = 0; q < pr.rank; ++q) {
+    pr.odiv[q] = make_fdiv((uint32_t)od[q]);
+    pr.istride[q] = os[q];
+  }
+  // tiled path: I = the input's leading dims (by input stride) until their product
+  // reaches 64; O = the output's leading dims, up to the first one in I, until 64
+  const int R = pr.rank;
+  std::vector<int64_t> ost(R);
+  {
+    int64_t acc = 1;
+    for (int q = 0; q < R; ++q) { ost[q] = acc; acc *= od[q]; }
+  }
+  std::vector<int> byin(R);
+  for (int q = 0; q < R; ++q) byin[q] = q;
+  std::sort(byin.begin(), byin.end(), [&](int u, int v) { return os[u] < os[v]; });
+  std::vector<int> gi, go;
+  int64_t pb = 1, pa = 1;
+  // choose how many leading input dims form I (then O = the leading output dims before the
+  // first dim of I): maximise the shorter of the contiguous read run (along b, at most 64)
+  // and the contiguous write run (along a, at most what fills a 4096-element tile)
+  int best_score = -1;
+  for (int ni = 1; ni <= std::min(R, GP_MAXG); ++ni) {
+    std::vector<int> ci(byin.begin(), byin.begin() + ni);
+    int64_t cb = 1;
+    for (int q : ci) cb *= od[q];
+    std::vector<int> co;
+    int64_t ca = 1;
+    for (int q = 0; q < R; ++q) {
+      if (ca >= 256 || (int)co.size() == GP_MAXG) break;
+      if (std::find(ci.begin(), ci.end(), q) != ci.end()) break;
+      co.push_back(q);
+      ca *= od[q];
+    }
+    const int64_t rb = std::min<int64_t>(cb, 64);
+    const int64_t ra = std::min<int64_t>(std::min<int64_t>(ca, 4096 / rb), 256);
+    const int score = co.empty() ? -1 : (int)std::min(rb, ra);
+    if (score > best_score) {
+      best_score = score;
+      gi = ci;
+      go = co;
+      pb = cb;
+      pa = ca;
+    }
+    if (cb >= 64) break;
+  }
+  // shared-dim tile: a dim S that follows a whole group I of leading input dims in the input
+  // and a whole group O of leading output dims in the output; a block takes all of I and O
+  // and a chunk of S, so its reads run over pb * tsc and its writes over pa * tsc elements
+  const int64_t scap = GP_SCAP_BYTES / (int64_t)elem_size(dt);
+  int sq = -1;
+  std::vector<int> s_gi;
+  int64_t s_pa = 0, s_pb = 0, s_tsc = 0, s_score = -1;
+  for (int q = 1; q < R; ++q) {
+    std::vector<int> ci;
+    int64_t cb = 1;
+    for (int u = 0; u < R; ++u)
+      if (os[u] < os[q]) { ci.push_back(u); cb *= od[u]; }
+    if (ci.empty() || cb != os[q]) continue;  // S must follow I contiguously in the input
+    bool overlap = false;
+    for (int u : ci)
+      if (u < q) overlap = true;               // I and O = output dims 0..q-1 disjoint
+    const int64_t ca = ost[q];
+    if (overlap || q > GP_MAXG || (int)ci.size() > GP_MAXG || ca > 256 || cb > 256 ||
+        ca * cb * 2 > scap)
+      continue;
+    std::sort(ci.begin(), ci.end(), [&](int u, int v) { return os[u] < os[v]; });
+    const int64_t tsc = std::min<int64_t>(od[q], scap / (ca * cb));
+    const int64_t score = std::min(ca, cb) * tsc;
+    if (score > s_score) {
+      s_score = score;
+      sq = q;
+      s_gi = ci;
+      s_pa = ca;
+      s_pb = cb;
+      s_tsc = tsc;
+    }
+  }
+  int64_t blocks;
+  if (sq >= 0 && s_score > best_score) {
+    pr.tiled = 2;
+    pr.no = sq;
+    pr.ni = (int32_t)s_gi.size();
+    pr.pa = s_pa;
+    pr.pb = s_pb;
+    pr.tla = (int32_t)s_pa;
+    pr.tlb = (int32_t)s_pb;
+    pr.tla_div = make_fdiv((uint32_t)s_pa);
+    pr.tlb_div = make_fdiv((uint32_t)s_pb);
+    pr.tsc = (int32_t)s_tsc;
+    pr.tsc_div = make_fdiv((uint32_t)s_tsc);
+    pr.s_ext = od[sq];
+    pr.s_is = os[sq];
+    pr.s_os = ost[sq];
+    pr.ts_tiles = (od[sq] + s_tsc - 1) / s_tsc;
+    for (int u = 0; u < sq; ++u) {
+      pr.odv[u] = make_fdiv((uint32_t)od[u]);
+      pr.o_is[u] = os[u];
+    }
+    for (int u = 0; u < pr.ni; ++u) {
+      pr.idv[u] = make_fdiv((uint32_t)od[s_gi[u]]);
+      pr.i_os[u] = ost[s_gi[u]];
+    }
+    int nr = 0;
+    int64_t restn = 1;
+    for (int q = 0; q < R; ++q) {
+      if (q <= sq || std::find(s_gi.begin(), s_gi.end(), q) != s_gi.end()) continue;
+      pr.rdv[nr] = make_fdiv((uint32_t)od[q]);
+      pr.r_is[nr] = os[q];
+      pr.r_os[nr] = ost[q];
+      restn *= od[q];
+      ++nr;
+    }
+    pr.nr = nr;
+    blocks = pr.ts_tiles * restn;
+  } else if (!gi.empty() && !go.empty() && os[gi[0]] == 1 && R > 1) {
+    pr.tiled = 1;
+    pr.no = (int32_t)go.size();
+    pr.ni = (int32_t)gi.size();
+    pr.pa = pa;
+    pr.pb = pb;
+    // tile: up to 64 along b, and along a as many as fill 4096 elements (at most 256)
+    const int64_t tlb = std::min<int64_t>(pb, 64);
+    const int64_t tla = std::min<int64_t>(std::min<int64_t>(pa, 4096 / tlb), 256);
+    pr.tla = (int32_t)tla;
+    pr.tlb = (int32_t)tlb;
+    pr.tla_div = make_fdiv((uint32_t)tla);
+    pr.tlb_div = make_fdiv((uint32_t)tlb);
+    pr.ta = (pa + tla - 1) / tla;
+    pr.tb = (pb + tlb - 1) / tlb;
+    for (int u = 0; u < pr.no; ++u) {
+      pr.odv[u] = make_fdiv((uint32_t)od[go[u]]);
+      pr.o_is[u] = os[go[u]];
+    }
+    for (int u = 0; u < pr.ni; ++u) {
+      pr.idv[u] = make_fdiv((uint32_t)od[gi[u]]);
+      pr.i_os[u] = ost[gi[u]];
+    }
+    int nr = 0;
+    int64_t restn = 1;
+    for (int q = 0; q < R; ++q) {
+      if (std::find(gi.begin(), gi.end(), q) != gi.end() ||
+          std::find(go.begin(), go.end(), q) != go.end())
+        continue;
+      pr.rdv[nr] = make_fdiv((uint32_t)od[q]);
+      pr.r_is[nr] = os[q];
+      pr.r_os[nr] = ost[q];
+      restn *= od[q];
+      ++nr;
+    }
+    pr.nr = nr;
+    blocks = pr.ta * pr.tb * restn;
+  } else {
+    blocks = (n + 1023) / 1024;
+  }
+  return blocks;
+}
+
 Status run_grouped_permute(ome_ctx* ctx, int dt, int np, const int32_t* ndims, const int64_t* dims,
                            const int32_t* perms, const void* const* a, void* const* c,
                            const Scal& sx, const Scal& sy, hipStream_t st) {
@@ -359,165 +526,9 @@ Status run_grouped_permute(ome_ctx* ctx, int dt, int np, const int32_t* ndims, c
     }
     if (od.empty()) { od.push_back(1); os.push_back(1); }
     if ((int)od.size() > GP_MAXR)
-      return Status::err(OME_ERR_UNSUPPORTED, "grouped permute of rank > 8 after grouping");
-    GPProb pr{};
-    pr.x = a[g];
-    pr.y = c[g];
-    pr.n = n;
-    pr.rank = (int32_t)od.size();
-    for (int q = 0; q < pr.rank; ++q) {
-      pr.odiv[q] = make_fdiv((uint32_t)od[q]);
-      pr.istride[q] = os[q];
-    }
-    // tiled path: I = the input's leading dims (by input stride) until their product
-    // reaches 64; O = the output's leading dims, up to the first one in I, until 64
-    const int R = pr.rank;
-    std::vector<int64_t> ost(R);
-    {
-      int64_t acc = 1;
-      for (int q = 0; q < R; ++q) { ost[q] = acc; acc *= od[q]; }
-    }
-    std::vector<int> byin(R);
-    for (int q = 0; q < R; ++q) byin[q] = q;
-    std::sort(byin.begin(), byin.end(), [&](int u, int v) { return os[u] < os[v]; });
-    std::vector<int> gi, go;
-    int64_t pb = 1, pa = 1;
-    // choose how many leading input dims form I (then O = the leading output dims before the
-    // first dim of I): maximise the shorter of the contiguous read run (along b, at most 64)
-    // and the contiguous write run (along a, at most what fills a 4096-element tile)
-    int best_score = -1;
-    for (int ni = 1; ni <= std::min(R, 4); ++ni) {
-      std::vector<int> ci(byin.begin(), byin.begin() + ni);
-      int64_t cb = 1;
-      for (int q : ci) cb *= od[q];
-      std::vector<int> co;
-      int64_t ca = 1;
-      for (int q = 0; q < R; ++q) {
-        if (ca >= 256 || co.size() == 4) break;
-        if (std::find(ci.begin(), ci.end(), q) != ci.end()) break;
-        co.push_back(q);
-        ca *= od[q];
-      }
-      const int64_t rb = std::min<int64_t>(cb, 64);
-      const int64_t ra = std::min<int64_t>(std::min<int64_t>(ca, 4096 / rb), 256);
-      const int score = co.empty() ? -1 : (int)std::min(rb, ra);
-      if (score > best_score) {
-        best_score = score;
-        gi = ci;
-        go = co;
-        pb = cb;
-        pa = ca;
-      }
-      if (cb >= 64) break;
-    }
-    // shared-dim tile: a dim S that follows a whole group I of leading input dims in the input
-    // and a whole group O of leading output dims in the output; a block takes all of I and O
-    // and a chunk of S, so its reads run over pb * tsc and its writes over pa * tsc elements
-    const int64_t scap = GP_SCAP_BYTES / (int64_t)elem_size(dt);
-    int sq = -1;
-    std::vector<int> s_gi;
-    int64_t s_pa = 0, s_pb = 0, s_tsc = 0, s_score = -1;
-    for (int q = 1; q < R; ++q) {
-      std::vector<int> ci;
-      int64_t cb = 1;
-      for (int u = 0; u < R; ++u)
-        if (os[u] < os[q]) { ci.push_back(u); cb *= od[u]; }
-      if (ci.empty() || cb != os[q]) continue;  // S must follow I contiguously in the input
-      bool overlap = false;
-      for (int u : ci)
-        if (u < q) overlap = true;               // I and O = output dims 0..q-1 disjoint
-      const int64_t ca = ost[q];
-      if (overlap || q > 4 || (int)ci.size() > 4 || ca > 256 || cb > 256 || ca * cb * 2 > scap)
-        continue;
-      std::sort(ci.begin(), ci.end(), [&](int u, int v) { return os[u] < os[v]; });
-      const int64_t tsc = std::min<int64_t>(od[q], scap / (ca * cb));
-      const int64_t score = std::min(ca, cb) * tsc;
-      if (score > s_score) {
-        s_score = score;
-        sq = q;
-        s_gi = ci;
-        s_pa = ca;
-        s_pb = cb;
-        s_tsc = tsc;
-      }
-    }
-    int64_t blocks;
-    if (sq >= 0 && s_score > best_score) {
-      pr.tiled = 2;
-      pr.no = sq;
-      pr.ni = (int32_t)s_gi.size();
-      pr.pa = s_pa;
-      pr.pb = s_pb;
-      pr.tla = (int32_t)s_pa;
-      pr.tlb = (int32_t)s_pb;
-      pr.tla_div = make_fdiv((uint32_t)s_pa);
-      pr.tlb_div = make_fdiv((uint32_t)s_pb);
-      pr.tsc = (int32_t)s_tsc;
-      pr.tsc_div = make_fdiv((uint32_t)s_tsc);
-      pr.s_ext = od[sq];
-      pr.s_is = os[sq];
-      pr.s_os = ost[sq];
-      pr.ts_tiles = (od[sq] + s_tsc - 1) / s_tsc;
-      for (int u = 0; u < sq; ++u) {
-        pr.odv[u] = make_fdiv((uint32_t)od[u]);
-        pr.o_is[u] = os[u];
-      }
-      for (int u = 0; u < pr.ni; ++u) {
-        pr.idv[u] = make_fdiv((uint32_t)od[s_gi[u]]);
-        pr.i_os[u] = ost[s_gi[u]];
-      }
-      int nr = 0;
-      int64_t restn = 1;
-      for (int q = 0; q < R; ++q) {
-        if (q <= sq || std::find(s_gi.begin(), s_gi.end(), q) != s_gi.end()) continue;
-        pr.rdv[nr] = make_fdiv((uint32_t)od[q]);
-        pr.r_is[nr] = os[q];
-        pr.r_os[nr] = ost[q];
-        restn *= od[q];
-        ++nr;
-      }
-      pr.nr = nr;
-      blocks = pr.ts_tiles * restn;
-    } else if (!gi.empty() && !go.empty() && os[gi[0]] == 1 && R > 1) {
-      pr.tiled = 1;
-      pr.no = (int32_t)go.size();
-      pr.ni = (int32_t)gi.size();
-      pr.pa = pa;
-      pr.pb = pb;
-      // tile: up to 64 along b, and along a as many as fill 4096 elements (at most 256)
-      const int64_t tlb = std::min<int64_t>(pb, 64);
-      const int64_t tla = std::min<int64_t>(std::min<int64_t>(pa, 4096 / tlb), 256);
-      pr.tla = (int32_t)tla;
-      pr.tlb = (int32_t)tlb;
-      pr.tla_div = make_fdiv((uint32_t)tla);
-      pr.tlb_div = make_fdiv((uint32_t)tlb);
-      pr.ta = (pa + tla - 1) / tla;
-      pr.tb = (pb + tlb - 1) / tlb;
-      for (int u = 0; u < pr.no; ++u) {
-        pr.odv[u] = make_fdiv((uint32_t)od[go[u]]);
-        pr.o_is[u] = os[go[u]];
-      }
-      for (int u = 0; u < pr.ni; ++u) {
-        pr.idv[u] = make_fdiv((uint32_t)od[gi[u]]);
-        pr.i_os[u] = ost[gi[u]];
-      }
-      int nr = 0;
-      int64_t restn = 1;
-      for (int q = 0; q < R; ++q) {
-        if (std::find(gi.begin(), gi.end(), q) != gi.end() ||
-            std::find(go.begin(), go.end(), q) != go.end())
-          continue;
-        pr.rdv[nr] = make_fdiv((uint32_t)od[q]);
-        pr.r_is[nr] = os[q];
-        pr.r_os[nr] = ost[q];
-        restn *= od[q];
-        ++nr;
-      }
-      pr.nr = nr;
-      blocks = pr.ta * pr.tb * restn;
-    } else {
-      blocks = (n + 1023) / 1024;
-    }
+      return Status::err(OME_ERR_UNSUPPORTED, "grouped permute of rank > 24 after grouping");
+    GPProb pr;
+    const int64_t blocks = gp_plan(dt, od, os, a[g], c[g], n, pr);
     probs.push_back(pr);
     first.push_back(blocks);  // block count for now; turned into first blocks below
   }

@@ -122,7 +122,8 @@ enum GGVariant : int {
 void gg_tile_shape(int dt, int variant, int& tm, int& tn);
 void launch_grouped_gemm(int dt, int variant, const GGDesc& g, int grid, hipStream_t st);
 
-constexpr int GP_MAXR = 8;
+constexpr int GP_MAXR = 24;
+constexpr int GP_MAXG = 8;  // dims in a permute tile's input or output group
 constexpr int GP_SCAP_BYTES = 32768;  // LDS bytes of a shared-dim permute tile
 // One permute.  Plain path (tiled == 0): output dims (fastest first) with their input strides.
 // Tiled path: the leading output dims (group O, linear index a, contiguous in y) and the
@@ -142,10 +143,10 @@ struct GPProb {
   int64_t ta, tb;            // tiled: tiles along a and b
   int32_t tla, tlb;          // tiled: tile extent along a and b (tla * tlb <= 4096)
   FDiv tlb_div, tla_div;
-  FDiv odv[4];               // O dims (output order), their input strides
-  int64_t o_is[4];
-  FDiv idv[4];               // I dims (input order), their output strides
-  int64_t i_os[4];
+  FDiv odv[GP_MAXG];         // O dims (output order), their input strides
+  int64_t o_is[GP_MAXG];
+  FDiv idv[GP_MAXG];         // I dims (input order), their output strides
+  int64_t i_os[GP_MAXG];
   FDiv rdv[GP_MAXR];         // R dims, input and output strides
   int64_t r_is[GP_MAXR], r_os[GP_MAXR];
   // tiled == 2: the whole of I and O and a chunk of tsc values of the shared dim S, the dim

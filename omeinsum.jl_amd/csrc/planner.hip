@@ -408,7 +408,8 @@ static std::vector<int> order_in(const std::vector<int>& grp, const std::vector<
 // a binary step goes through the strided kernel when its contracted space has at most this
 // many points and output points x contracted points stays below the work bound
 constexpr int64_t STRIDED_MAX_K = 64;
-constexpr int64_t STRIDED_MAX_WORK = 1 << 24;
+constexpr int64_t STRIDED_MAX_OUT = 1 << 16;  // larger outputs: tiled permutes + GEMM stream better
+constexpr int64_t STRIDED_MAX_WORK = 1 << 20;
 
 static Status do_binary(ome_ctx* ctx, const EinsumCall& c, const std::vector<int>& ix1,
                         const std::vector<int>& ix2, const std::vector<int>& iy, const void* x1,
@@ -489,7 +490,8 @@ static Status do_binary(ome_ctx* ctx, const EinsumCall& c, const std::vector<int
     const int64_t nk = prod_sizes(in, sz);
     const bool all_inplace = best.a_inplace && best.b_inplace && best.y_inplace;
     if (ctx->opt.gemm_kernel == GEMM_AUTO && !all_inplace && !a_pre && !b_pre && !y_post &&
-        nk <= STRIDED_MAX_K && nyy * nk <= STRIDED_MAX_WORK && (int)in.size() <= SG_MAXK &&
+        nk <= STRIDED_MAX_K && nyy <= STRIDED_MAX_OUT && nyy * nk <= STRIDED_MAX_WORK &&
+        (int)in.size() <= SG_MAXK &&
         (int)iy.size() <= SG_MAXO && na < (1ll << 31) && nb < (1ll << 31) && nyy < (1ll << 31)) {
       auto stride_in = [&](const std::vector<int>& ls, int l) -> int32_t {
         int64_t s = 1;

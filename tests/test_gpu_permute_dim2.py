@@ -1,0 +1,62 @@
+"""GPU parity of large single permutes whose dims, after merging consecutive runs, stay short
+(tensor-network labels of extent 2, ranks 15-20): tensorpermute! (src/utils.jl:122-156)
+through the C ABI, exact against numpy for every element size, with and without scaling.
+"""
+import numpy as np
+import pytest
+
+from helpers import rng
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def om():
+    import omeinsum_mi355x
+    return omeinsum_mi355x
+
+
+def values(r, shape, dtype):
+    if dtype == np.bool_:
+        return r.random(shape) < 0.5
+    v = r.integers(-8, 9, shape)
+    if np.dtype(dtype).kind == "c":
+        v = v + 1j * r.integers(-8, 9, shape)
+    return v.astype(dtype)
+
+
+CASES = [(15, 0), (16, 1), (18, 2), (17, 3), (20, 4)]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.complex64, np.complex128, np.int64,
+                                   np.bool_])
+@pytest.mark.parametrize("rank,seed", CASES)
+def test_dim2_permutes(dtype, rank, seed):
+    o = om()
+    r = rng(4100 + seed)
+    shape = [2] * rank
+    if seed == 4:
+        shape[3] = 3  # one longer dim
+        shape[11] = 5
+    perm = [int(p) for p in r.permutation(rank)]
+    x = values(r, shape, dtype)
+    ref = np.transpose(x, perm)
+    t = torch.from_numpy(x).cuda()
+    y = torch.empty(ref.shape, dtype=t.dtype, device="cuda")
+    if dtype == np.bool_:
+        t, y = o.TropicalAndOr(t), o.TropicalAndOr(y)
+    o.tensorpermute(y, t, perm)
+    torch.cuda.synchronize()
+    assert np.array_equal(o.to_numpy(y), ref)
+    if dtype not in (np.bool_,):
+        y0 = values(r, ref.shape, dtype)
+        yt = torch.from_numpy(np.ascontiguousarray(y0)).cuda()
+        o.tensorpermute(yt, t, perm, 3, -2)
+        torch.cuda.synchronize()
+        assert np.array_equal(o.to_numpy(yt), 3 * ref - 2 * y0)
