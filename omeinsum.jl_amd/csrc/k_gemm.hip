@@ -160,6 +160,53 @@ __global__ __launch_bounds__(256) void k_gemm_strided(StridedGemmDesc d) {
   }
 }
 
+template <class O>
+__global__ __launch_bounds__(256) void k_fused_steps(const FusedStep* __restrict__ steps,
+                                                     const int32_t* __restrict__ first,
+                                                     FusedLeaves lv, char* arena) {
+  using T = typename O::T;
+  __shared__ int32_t ska[SF_MAXNK], skb[SF_MAXNK];
+  const int s1 = first[blockIdx.x + 1];
+  for (int s = first[blockIdx.x]; s < s1; ++s) {
+    const FusedStep& d = steps[s];
+    const T* __restrict__ A = d.a_leaf >= 0 ? (const T*)lv.p[d.a_leaf] : (const T*)(arena + d.a_off);
+    const T* __restrict__ B = d.b_leaf >= 0 ? (const T*)lv.p[d.b_leaf] : (const T*)(arena + d.b_off);
+    T* __restrict__ Y = (T*)(arena + d.y_off);
+    const int nk = d.nk;
+    for (int k = threadIdx.x; k < nk; k += 256) {
+      uint32_t r = (uint32_t)k;
+      int32_t oa = 0, ob = 0;
+      for (int q = 0; q < d.ni; ++q) {
+        const uint32_t qq = fdiv_q(r, d.kdiv[q]);
+        const int32_t v = (int32_t)(r - qq * d.kdiv[q].d);
+        oa += v * d.ka[q];
+        ob += v * d.kb[q];
+        r = qq;
+      }
+      ska[k] = oa;
+      skb[k] = ob;
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < d.nout; o += 256) {
+      uint32_t r = (uint32_t)o;
+      int32_t pa = 0, pb = 0, py = 0;
+      for (int q = 0; q < d.no; ++q) {
+        const uint32_t qq = fdiv_q(r, d.odiv[q]);
+        const int32_t v = (int32_t)(r - qq * d.odiv[q].d);
+        pa += v * d.oa[q];
+        pb += v * d.ob[q];
+        py += v * d.oy[q];
+        r = qq;
+      }
+      T acc = O::zero();
+      for (int k = 0; k < nk; ++k) acc = O::fma(A[pa + ska[k]], B[pb + skb[k]], acc);
+      Y[py] = acc;
+    }
+    // this step's outputs are the next steps' operands; the offset table is rewritten next
+    __syncthreads();
+  }
+}
+
 // One thread per output element, K walked directly from global memory: for tiny M/N or K
 // (tensor-network steps over dimension-2 labels) where a 64x64 tile would idle.
 template <class O>
@@ -297,6 +344,17 @@ void launch_gemm_strided(int dt, const StridedGemmDesc& d, hipStream_t st) {
     StridedGemmDesc dd = d;
     dd.mode = epi_mode<O>(d.sx, d.sy);
     hipLaunchKernelGGL(k_gemm_strided<O>, dim3((uint32_t)g), dim3(256), 0, st, dd);
+  });
+}
+
+void launch_fused_steps(int dt, const FusedStep* steps, const int32_t* first, int nsub,
+                        const FusedLeaves& leaves, char* arena, hipStream_t st) {
+  if (nsub <= 0) return;
+  OME_DRY_RETURN(1);
+  TimedScope ts(KC_GEMM_VALU, st);
+  OME_DISPATCH(dt, O, {
+    hipLaunchKernelGGL(k_fused_steps<O>, dim3((uint32_t)nsub), dim3(256), 0, st, steps, first,
+                       leaves, arena);
   });
 }
 

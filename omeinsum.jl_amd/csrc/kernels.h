@@ -78,6 +78,30 @@ struct StridedGemmDesc {
 };
 void launch_gemm_strided(int dt, const StridedGemmDesc& d, hipStream_t st);
 
+// Small tree steps fused into one launch: workgroup b evaluates steps [first[b], first[b+1])
+// of a subtree in order (each the strided contraction above, outputs overwritten), with a
+// workgroup barrier between steps.  Operands are leaves (pointers passed with the launch) or
+// earlier outputs in the plan's arena.
+constexpr int SF_MAXO = 16;      // output labels of a fused step
+constexpr int SF_MAXK = 8;       // contracted labels
+constexpr int SF_MAXNK = 64;     // contracted points
+constexpr int SF_MAXOUT = 8192;  // output points
+constexpr int SF_MAXLEAF = 256;  // leaves of a tree whose steps may be fused
+struct FusedStep {
+  int32_t a_leaf, b_leaf;        // >= 0: leaf index; < 0: the operand is in the arena
+  int64_t a_off, b_off, y_off;   // arena byte offsets
+  int32_t nout, nk, no, ni;
+  FDiv odiv[SF_MAXO];            // output labels in y's order (first fastest)
+  int32_t oa[SF_MAXO], ob[SF_MAXO], oy[SF_MAXO];
+  FDiv kdiv[SF_MAXK];            // contracted labels
+  int32_t ka[SF_MAXK], kb[SF_MAXK];
+};
+struct FusedLeaves {
+  const void* p[SF_MAXLEAF];
+};
+void launch_fused_steps(int dt, const FusedStep* steps, const int32_t* first, int nsub,
+                        const FusedLeaves& leaves, char* arena, hipStream_t st);
+
 // Few outputs (M*N*batch <= 1024), long K: one workgroup per output and K part.
 int dot_plan_parts(const GemmDesc& d);
 void launch_gemm_dot(int dt, GemmDesc d, int parts, hipStream_t st);

@@ -489,8 +489,10 @@ static Status do_binary(ome_ctx* ctx, const EinsumCall& c, const std::vector<int
   {
     const int64_t nk = prod_sizes(in, sz);
     const bool all_inplace = best.a_inplace && best.b_inplace && best.y_inplace;
-    if (ctx->opt.gemm_kernel == GEMM_AUTO && !all_inplace && !a_pre && !b_pre && !y_post &&
-        nk <= STRIDED_MAX_K && nyy <= STRIDED_MAX_OUT && nyy * nk <= STRIDED_MAX_WORK &&
+    // (Float32 / ComplexF32 at FP64 precision keep the widened FP64 GEMM: they round once)
+    const bool fp64_widened = ctx->opt.precision == PREC_FP64 && (dt == F32 || dt == C32);
+    if (ctx->opt.gemm_kernel == GEMM_AUTO && !fp64_widened && !all_inplace && !a_pre && !b_pre &&
+        !y_post && nk <= STRIDED_MAX_K && nyy <= STRIDED_MAX_OUT && nyy * nk <= STRIDED_MAX_WORK &&
         (int)in.size() <= SG_MAXK &&
         (int)iy.size() <= SG_MAXO && na < (1ll << 31) && nb < (1ll << 31) && nyy < (1ll << 31)) {
       auto stride_in = [&](const std::vector<int>& ls, int l) -> int32_t {

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <functional>
 #include <list>
 #include <map>
 
@@ -129,6 +130,11 @@ struct ome_tree {
   // gradient plan (lazily built)
   char* gmem = nullptr;
   int64_t g_arena = 0, g_scratch = 0;
+  // small steps fused into one launch (one workgroup per subtree of small steps)
+  std::vector<char> fused;           // per step
+  FusedStep* fdev = nullptr;         // device descriptors, subtree by subtree
+  int32_t* ffirst = nullptr;         // device: first descriptor of each subtree (+ end)
+  int fsub = 0;                      // subtrees
 };
 
 namespace {
@@ -185,14 +191,23 @@ std::vector<int> drop_sliced(const std::vector<int>& ls, const std::vector<int>&
 }  // namespace
 
 // Plans the arena: every step output placed first-fit; children released after the parent.
+// Fused steps (all evaluated by one launch before the other steps) are placed first: the
+// outputs inside a fused subtree keep their blocks for the whole evaluation and a subtree's
+// root output is released after its (unfused) parent, so the plan also holds when every step
+// runs one by one in order (the normalised evaluation).
 static void plan_arena(ome_tree* t, bool keep_all, int64_t* out_bytes) {
   Arena ar;
   const size_t es = elem_size(t->dt);
   const int n = (int)t->steps.size();
+  auto is_fused = [&](int s) { return !t->fused.empty() && t->fused[s]; };
   for (int s = 0; s < n; ++s) {
     TreeStep& st = t->steps[s];
-    std::vector<int> oy = drop_sliced(st.iy, t->slicing);
-    st.out_elems = prod(oy, t->sizes_sl);
+    st.out_elems = prod(drop_sliced(st.iy, t->slicing), t->sizes_sl);
+    if (is_fused(s)) st.out_off = ar.place(st.out_elems * es);
+  }
+  for (int s = 0; s < n; ++s) {
+    if (is_fused(s)) continue;
+    TreeStep& st = t->steps[s];
     // the root of an unsliced, non-scalar-mode tree writes straight into y; it still gets
     // a block so scalar/normalised and sliced evaluation have somewhere to write
     st.out_off = ar.place(st.out_elems * es);
@@ -203,6 +218,125 @@ static void plan_arena(ome_tree* t, bool keep_all, int64_t* out_bytes) {
   *out_bytes = align256(ar.high);
 }
 
+// Marks the steps to fuse: binary, not the root, unique labels on both operands and the
+// output with none dangling or broadcast (after dropping sliced labels), a short contracted
+// space and a small output; fused are the maximal subtrees made only of such steps.
+static void plan_fusion(ome_tree* t) {
+  const int n = (int)t->steps.size();
+  t->fused.assign(n, 0);
+  if (t->opt.gemm_kernel != GEMM_AUTO || t->n_leaves > SF_MAXLEAF ||
+      (t->opt.precision == PREC_FP64 && (t->dt == F32 || t->dt == C32)))
+    return;
+  auto unique = [](const std::vector<int>& v) {
+    for (size_t i = 0; i < v.size(); ++i)
+      for (size_t j = i + 1; j < v.size(); ++j)
+        if (v[i] == v[j]) return false;
+    return true;
+  };
+  std::vector<char> ok(n, 0);
+  for (int s = 0; s + 1 < n; ++s) {
+    const TreeStep& st = t->steps[s];
+    if (st.args.size() != 2) continue;
+    const std::vector<int> x1 = drop_sliced(st.args[0].ix, t->slicing);
+    const std::vector<int> x2 = drop_sliced(st.args[1].ix, t->slicing);
+    const std::vector<int> y = drop_sliced(st.iy, t->slicing);
+    if (!unique(x1) || !unique(x2) || !unique(y)) continue;
+    bool cover = true;
+    std::vector<int> inner;
+    for (int l : x1) {
+      if (!has(x2, l) && !has(y, l)) cover = false;
+      if (has(x2, l) && !has(y, l)) inner.push_back(l);
+    }
+    for (int l : x2)
+      if (!has(x1, l) && !has(y, l)) cover = false;
+    for (int l : y)
+      if (!has(x1, l) && !has(x2, l)) cover = false;
+    if (!cover) continue;
+    const int64_t nout = prod(y, t->sizes_sl), nk = prod(inner, t->sizes_sl);
+    ok[s] = nout <= SF_MAXOUT && nk <= SF_MAXNK && nout * nk <= (1 << 15) &&
+            (int)y.size() <= SF_MAXO && (int)inner.size() <= SF_MAXK;
+  }
+  int count = 0;
+  for (int s = 0; s < n; ++s) {  // post-order: children first
+    bool f = ok[s];
+    for (auto& a : t->steps[s].args)
+      if (a.ref < 0 && !t->fused[-(a.ref + 1)]) f = false;
+    t->fused[s] = f;
+    count += f;
+  }
+  if (count < 2) t->fused.assign(n, 0);  // nothing to gain from one fused step
+}
+
+// Device descriptors of the fused subtrees (after plan_arena has placed their outputs).
+static Status upload_fusion(ome_tree* t) {
+  const int n = (int)t->steps.size();
+  std::vector<FusedStep> desc;
+  std::vector<int32_t> first;
+  std::function<void(int)> add = [&](int s) {
+    for (auto& a : t->steps[s].args)
+      if (a.ref < 0) add(-(a.ref + 1));
+    const TreeStep& st = t->steps[s];
+    const std::vector<int> x1 = drop_sliced(st.args[0].ix, t->slicing);
+    const std::vector<int> x2 = drop_sliced(st.args[1].ix, t->slicing);
+    const std::vector<int> y = drop_sliced(st.iy, t->slicing);
+    auto stride_in = [&](const std::vector<int>& ls, int l) -> int32_t {
+      int64_t v = 1;
+      for (int l2 : ls) {
+        if (l2 == l) return (int32_t)v;
+        v *= t->sizes_sl[l2];
+      }
+      return 0;
+    };
+    FusedStep d{};
+    const TreeArg& a0 = st.args[0];
+    const TreeArg& a1 = st.args[1];
+    d.a_leaf = a0.ref >= 0 ? a0.ref : -1;
+    d.b_leaf = a1.ref >= 0 ? a1.ref : -1;
+    d.a_off = a0.ref >= 0 ? 0 : t->steps[-(a0.ref + 1)].out_off;
+    d.b_off = a1.ref >= 0 ? 0 : t->steps[-(a1.ref + 1)].out_off;
+    d.y_off = st.out_off;
+    std::vector<int> inner;
+    for (int l : x1)
+      if (has(x2, l) && !has(y, l)) inner.push_back(l);
+    d.nout = (int32_t)prod(y, t->sizes_sl);
+    d.nk = (int32_t)prod(inner, t->sizes_sl);
+    d.no = (int32_t)y.size();
+    d.ni = (int32_t)inner.size();
+    for (int q = 0; q < d.no; ++q) {
+      d.odiv[q] = make_fdiv((uint32_t)t->sizes_sl[y[q]]);
+      d.oa[q] = stride_in(x1, y[q]);
+      d.ob[q] = stride_in(x2, y[q]);
+      d.oy[q] = stride_in(y, y[q]);
+    }
+    for (int q = 0; q < d.ni; ++q) {
+      d.kdiv[q] = make_fdiv((uint32_t)t->sizes_sl[inner[q]]);
+      d.ka[q] = stride_in(x1, inner[q]);
+      d.kb[q] = stride_in(x2, inner[q]);
+    }
+    desc.push_back(d);
+  };
+  for (int s = 0; s < n; ++s) {
+    const int parent = t->steps[s].parent;
+    if (!t->fused[s] || (parent >= 0 && t->fused[parent])) continue;  // subtree roots only
+    first.push_back((int32_t)desc.size());
+    add(s);
+  }
+  first.push_back((int32_t)desc.size());
+  t->fsub = (int)first.size() - 1;
+  if (t->fsub <= 0) return Status::ok();
+  ome_ctx* ctx = t->ctx;
+  t->fdev = (FusedStep*)ctx->pool->alloc(desc.size() * sizeof(FusedStep));
+  t->ffirst = (int32_t*)ctx->pool->alloc(first.size() * sizeof(int32_t));
+  if (!t->fdev || !t->ffirst) return Status::err(OME_ERR_OOM, "device pool exhausted (fused steps)");
+  if (hipMemcpyAsync(t->fdev, desc.data(), desc.size() * sizeof(FusedStep), hipMemcpyHostToDevice,
+                     ctx->stream) != hipSuccess ||
+      hipMemcpyAsync(t->ffirst, first.data(), first.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                     ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess)
+    return Status::err(OME_ERR_DEVICE, "upload of the fused step descriptors failed");
+  return Status::ok();
+}
+
 // Enqueue one evaluation of the (reduced) tree for one slice.  xs are the leaf pointers to
 // use (staged copies for sliced leaves); the root writes into `root_out` with rsx/rsy.
 static Status run_steps(ome_tree* t, const std::vector<const void*>& xs, void* root_out,
@@ -211,7 +345,15 @@ static Status run_steps(ome_tree* t, const std::vector<const void*>& xs, void* r
   ome_ctx* ctx = t->ctx;
   const Scal one = scal_one(t->dt), zero = scal_zero(t->dt);
   const int n = (int)t->steps.size();
+  // the fused subtrees first, in one launch (the normalised evaluation runs every step alone)
+  const bool fuse = t->fsub > 0 && !normalize;
+  if (fuse) {
+    FusedLeaves lv{};
+    for (int k = 0; k < t->n_leaves; ++k) lv.p[k] = xs[k];
+    launch_fused_steps(t->dt, t->fdev, t->ffirst, t->fsub, lv, t->mem, st);
+  }
   for (int s = 0; s < n; ++s) {
+    if (fuse && t->fused[s]) continue;
     TreeStep& step = t->steps[s];
     EinsumCall c;
     c.dt = t->dt;
@@ -444,7 +586,17 @@ int ome_tree_create(ome_ctx* ctx, int dtype, const int32_t* desc, int64_t desc_l
   for (int l : t->slicing) t->sizes_sl[l] = 1;
   const size_t es = elem_size(dtype);
   // ---- memory plan
+  plan_fusion(t);
   plan_arena(t, false, &t->arena_bytes);
+  {
+    Status f = upload_fusion(t);
+    if (!f.good()) {
+      if (t->fdev) ctx->pool->free(t->fdev);
+      if (t->ffirst) ctx->pool->free(t->ffirst);
+      delete t;
+      return tree_fail(ctx, f);
+    }
+  }
   t->slice_out_off = t->steps.back().out_off;
   t->stage_off.assign(nleaves, -1);
   int64_t stage = 0;
@@ -475,7 +627,12 @@ int ome_tree_create(ome_ctx* ctx, int dtype, const int32_t* desc, int64_t desc_l
                          ctx->stream, &bump, false, nullptr, nullptr);
     lm.dry = false;
     t->mem = nullptr;
-    if (!r.good()) { delete t; return tree_fail(ctx, r); }
+    if (!r.good()) {
+      if (t->fdev) ctx->pool->free(t->fdev);
+      if (t->ffirst) ctx->pool->free(t->ffirst);
+      delete t;
+      return tree_fail(ctx, r);
+    }
     t->scratch_bytes = align256((int64_t)bump.high + 256);
     t->info.n_kernels = lm.count * t->nslices +
                         (t->slicing.empty() ? 0 : t->nslices * 2 + 2);
@@ -489,6 +646,8 @@ int ome_tree_create(ome_ctx* ctx, int dtype, const int32_t* desc, int64_t desc_l
   if (!ctx->opt.defer_tree_mem) {
     t->mem = (char*)ctx->pool->alloc((size_t)total);
     if (!t->mem) {
+      if (t->fdev) ctx->pool->free(t->fdev);
+      if (t->ffirst) ctx->pool->free(t->ffirst);
       delete t;
       return tree_fail(ctx, Status::err(OME_ERR_OOM, "device pool exhausted (tree arena of " +
                                                          std::to_string(total) + " bytes)"));
@@ -796,6 +955,8 @@ int ome_tree_destroy(ome_tree* t) {
   for (auto& ge : t->graphs) hipGraphExecDestroy(ge.exec);
   if (t->cap_stream) hipStreamDestroy(t->cap_stream);
   if (t->mem && t->owns_mem) ctx->pool->free(t->mem);
+  if (t->fdev) ctx->pool->free(t->fdev);
+  if (t->ffirst) ctx->pool->free(t->ffirst);
   delete t;
   return OME_OK;
 }

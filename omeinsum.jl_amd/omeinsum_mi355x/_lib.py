@@ -175,6 +175,8 @@ class Context:
         self._opts[key] = value
 
     def get_option(self, key):
+        if key in self._opts:  # set through this object: the library holds that value
+            return self._opts[key]
         v = c_i64()
         check(self.lib.ome_get_option(self.h, key, ctypes.byref(v)), self.h)
         return v.value

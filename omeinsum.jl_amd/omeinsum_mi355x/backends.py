@@ -133,7 +133,8 @@ class MI355XBackend(EinsumBackend):
         if isinstance(code, EinCode):
             return E.flat_einsum(code, xs, size_dict)
         from .tree import plan_for
-        return plan_for(code, xs, size_dict).execute(xs)
+        plan, ts = plan_for(code, xs, size_dict, with_tensors=True)
+        return plan.execute(xs, ts=ts)
 
     def einsum_inplace(self, code, xs, y, sx, sy, size_dict):
         from . import ein_ops as E
@@ -141,7 +142,8 @@ class MI355XBackend(EinsumBackend):
         if isinstance(code, EinCode):
             return E._flat_einsum_inplace(code.ixs, code.iy, xs, y, sx, sy, size_dict)
         from .tree import plan_for
-        return plan_for(code, xs, size_dict).execute(xs, y=y, sx=sx, sy=sy)
+        plan, ts = plan_for(code, xs, size_dict, with_tensors=True)
+        return plan.execute(xs, y=y, sx=sx, sy=sy, ts=ts)
 
 
 _BACKEND = [DefaultBackend()]

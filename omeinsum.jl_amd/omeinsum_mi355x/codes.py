@@ -256,7 +256,12 @@ class NestedEinsum:
                 and all(a == b for a, b in zip(self.args, other.args)))
 
     def __hash__(self):
-        return hash(self.to_plain_key())
+        # a tree is not modified after construction: its hash is computed once
+        h = self.__dict__.get("_hash")
+        if h is None:
+            h = hash(self.to_plain_key())
+            self.__dict__["_hash"] = h
+        return h
 
     def to_plain_key(self):
         if self.isleaf():
@@ -300,7 +305,11 @@ class NestedEinsum:
         return list(self.eins.iy)
 
     def uniquelabels(self):
-        return _unique([l for ix in self.getixsv() for l in ix] + self.getiyv())
+        ul = self.__dict__.get("_labels")
+        if ul is None:
+            ul = _unique([l for ix in self.getixsv() for l in ix] + self.getiyv())
+            self.__dict__["_labels"] = ul
+        return list(ul)
 
     def labeltype(self):
         return EinCode(self.getixsv(), self.getiyv()).labeltype()

@@ -253,8 +253,8 @@ def to_device(x, device=None):
     if device is None:
         cur = torch.cuda.current_device()
         # already a contiguous tensor of a supported type on the current device: as it is
-        if isinstance(x, torch.Tensor) and x.is_cuda and x.device.index == cur and \
-                x.dtype in _AS_IS and x.is_contiguous():
+        if type(x) is torch.Tensor and x.get_device() == cur and x.dtype in _AS_IS and \
+                x.is_contiguous():
             return x
         device = torch.device("cuda", cur)
     if isinstance(x, torch.Tensor):

@@ -88,11 +88,10 @@ class TreePlan:
         _lib.check(self.ctx.lib.ome_get_tree_info(self.h, ctypes.byref(info)), self.ctx.h)
         return info
 
-    def _ptrs(self, xs):
-        ts = []
-        for x in xs:
-            t = to_device(raw(x))
-            ts.append(t)
+    def _ptrs(self, xs, ts=None):
+        """Device tensors of xs (ts when the caller has them already) and their pointers."""
+        if ts is None:
+            ts = [to_device(raw(x)) for x in xs]
         return ts, (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
 
     def _out(self, ts, kind):
@@ -100,11 +99,11 @@ class TreePlan:
         y = torch.empty(self.out_shape, dtype=ts[0].dtype, device=ts[0].device)
         return (wrap(kind, y)), y
 
-    def execute(self, xs, y=None, sx=True, sy=False):
-        """einsum / einsum! of the tree on the device."""
+    def execute(self, xs, y=None, sx=True, sy=False, ts=None):
+        """einsum / einsum! of the tree on the device (ts: xs already as device tensors)."""
         self.ctx.sync_stream()
         kind = common_kind(xs)
-        ts, ptrs = self._ptrs(xs)
+        ts, ptrs = self._ptrs(xs, ts)
         if y is None:
             res, yt = self._out(ts, kind)
         else:
@@ -154,16 +153,15 @@ def element_code(xs):
     """The C ABI element type of a tree's inputs (they must share one type)."""
     kind = common_kind(xs)
     ts = [to_device(raw(x)) for x in xs]
-    import torch
-    T = ts[0].dtype
-    for t in ts[1:]:
-        T = torch.promote_types(T, t.dtype)
-    if any(t.dtype != T for t in ts):
+    dts = {t.dtype for t in ts}
+    if len(dts) != 1:
         raise TypeError("tree inputs must share one element type (promote first)")
-    return dtype_code(T, kind), ts
+    return dtype_code(dts.pop(), kind), ts
 
 
-def plan_for(code, xs, size_dict):
+def plan_for(code, xs, size_dict, with_tensors=False):
+    """The cached plan of code for inputs like xs; with_tensors=True also returns xs as device
+    tensors (for execute(..., ts=...))."""
     code_dt, ts = element_code(xs)
     labels = code.uniquelabels() if not isinstance(code, SlicedEinsum) else code.eins.uniquelabels()
     # a plan freezes the context's planning options when it is created
@@ -178,4 +176,4 @@ def plan_for(code, xs, size_dict):
             _CACHE.popitem(last=False)
     else:
         _CACHE.move_to_end(key)
-    return plan
+    return (plan, ts) if with_tensors else plan
