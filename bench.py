@@ -23,7 +23,7 @@ program omeinsum.jl_amd/calib/mfma_loop on random operands; the matrix cores dra
 on random data and the part lowers its clock, so this is the rate the loop can reach on real
 data, and frac there is achieved / that rate.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|c4|c3|c5]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|c4|c1|c2|c3|c5]
                        [--no-cpu] [--copy-calibration]
 """
 import argparse
@@ -612,7 +612,7 @@ def main():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", choices=("all", "c4", "c3", "c5"), default="all")
+    p.add_argument("--workload", choices=("all", "c4", "c1", "c2", "c3", "c5"), default="all")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--copy-calibration", action="store_true")
     args = p.parse_args()
@@ -632,7 +632,9 @@ def main():
     extra = {"c4": c4}
     if args.workload == "all":
         extra["c4_more"] = bench_c4_more(ab, args.steps, args.warmup)
+    if args.workload in ("all", "c1"):
         extra["c1_ijk_jkl_il_64"] = bench_c1(args.steps, args.warmup)
+    if args.workload in ("all", "c2"):
         extra["c2_network_100"] = bench_c2(max(5, args.steps // 2), 2)
     if args.workload in ("all", "c3"):
         extra["c3_grouped"] = bench_c3(max(3, args.steps // 4), 1)
