@@ -161,9 +161,9 @@ __global__ __launch_bounds__(256) void k_gemm_strided(StridedGemmDesc d) {
 }
 
 template <class O>
-__global__ __launch_bounds__(256) void k_fused_steps(const FusedStep* __restrict__ steps,
-                                                     const int32_t* __restrict__ first,
-                                                     FusedLeaves lv, char* arena) {
+__global__ __launch_bounds__(SF_THREADS) void k_fused_steps(const FusedStep* __restrict__ steps,
+                                                            const int32_t* __restrict__ first,
+                                                            FusedLeaves lv, char* arena) {
   using T = typename O::T;
   __shared__ int32_t ska[SF_MAXNK], skb[SF_MAXNK];
   const int s1 = first[blockIdx.x + 1];
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void k_fused_steps(const FusedStep* __restrict
     const T* __restrict__ B = d.b_leaf >= 0 ? (const T*)lv.p[d.b_leaf] : (const T*)(arena + d.b_off);
     T* __restrict__ Y = (T*)(arena + d.y_off);
     const int nk = d.nk;
-    for (int k = threadIdx.x; k < nk; k += 256) {
+    for (int k = threadIdx.x; k < nk; k += SF_THREADS) {
       uint32_t r = (uint32_t)k;
       int32_t oa = 0, ob = 0;
       for (int q = 0; q < d.ni; ++q) {
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_fused_steps(const FusedStep* __restrict
       skb[k] = ob;
     }
     __syncthreads();
-    for (int o = threadIdx.x; o < d.nout; o += 256) {
+    for (int o = threadIdx.x; o < d.nout; o += SF_THREADS) {
       uint32_t r = (uint32_t)o;
       int32_t pa = 0, pb = 0, py = 0;
       for (int q = 0; q < d.no; ++q) {
@@ -353,7 +353,7 @@ void launch_fused_steps(int dt, const FusedStep* steps, const int32_t* first, in
   OME_DRY_RETURN(1);
   TimedScope ts(KC_GEMM_VALU, st);
   OME_DISPATCH(dt, O, {
-    hipLaunchKernelGGL(k_fused_steps<O>, dim3((uint32_t)nsub), dim3(256), 0, st, steps, first,
+    hipLaunchKernelGGL(k_fused_steps<O>, dim3((uint32_t)nsub), dim3(SF_THREADS), 0, st, steps, first,
                        leaves, arena);
   });
 }

@@ -311,6 +311,139 @@ __global__ __launch_bounds__(256) void k_mfma_f32(GemmDesc d, int64_t tiles_m, i
   store_acc32<OpsF32>(d, acc, m0 + wm, n0 + wn, ti.batch, ti.ks, lane);
 }
 
+// ---- ComplexF32 on the FP32 matrix cores, realified while staged ---------------------------
+// Output tile 128 m x 64 n (complex), K-steps of 8 complex k (16 real).  The LDS tiles hold the
+// real operands of the four-real-products form, built from the complex operands as they are
+// staged (no operand preparation in HBM, no combine pass):
+//   A'[m][k] = Ar(m, k), A'[m][8 + k] = Ai(m, k);
+//   B' rows n' run over [re n 0..31 | im n 0..31 | re n 32..63 | im n 32..63]:
+//   B'[re n][k] = Br, B'[im n][k] = Bi, B'[re n][8 + k] = -Bi, B'[im n][8 + k] = Br,
+// so C' = A' B' holds Cr in the re rows and Ci in the im rows, and the two 32-wide blocks of a
+// wave are the real and imaginary parts of the same 32 x 32 outputs.
+constexpr int C32_BK = 8;
+
+// A tile: 128 rows x 8 k, 4 complex per thread.
+//   KMAJ (sk == 1): row t / 2, k = (t % 2) * 4 + i;  !KMAJ (sr == 1): row t % 128, k = (t / 128) * 4 + i
+template <bool KMAJ>
+__device__ __forceinline__ void load_c32_a(const c32* __restrict__ P, int64_t r0, int64_t k0,
+                                           int64_t R, int64_t K, int64_t sr, int64_t sk,
+                                           c32 (&v)[4]) {
+  const int t = threadIdx.x;
+  const int64_t r = r0 + (KMAJ ? (t >> 1) : (t & 127));
+  const int64_t kb = k0 + (KMAJ ? (t & 1) * 4 : (t >> 7) * 4);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const bool ok = r < R && kb + i < K;
+    v[i] = ok ? P[r * sr + (kb + i) * sk] : c32{0.f, 0.f};
+  }
+}
+
+// B tile: 64 rows (n) x 8 k, 2 complex per thread.
+//   KMAJ (sk == 1): row t / 4, k = (t % 4) * 2 + i;  !KMAJ (sn == 1): row t % 64, k = (t / 64) * 2 + i
+template <bool KMAJ>
+__device__ __forceinline__ void load_c32_b(const c32* __restrict__ P, int64_t r0, int64_t k0,
+                                           int64_t R, int64_t K, int64_t sr, int64_t sk,
+                                           c32 (&v)[2]) {
+  const int t = threadIdx.x;
+  const int64_t r = r0 + (KMAJ ? (t >> 2) : (t & 63));
+  const int64_t kb = k0 + (KMAJ ? (t & 3) * 2 : (t >> 6) * 2);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const bool ok = r < R && kb + i < K;
+    v[i] = ok ? P[r * sr + (kb + i) * sk] : c32{0.f, 0.f};
+  }
+}
+
+template <bool AK, bool BK>
+__global__ __launch_bounds__(256) void k_mfma_c32(GemmDesc d, int64_t tiles_m, int64_t tiles_n,
+                                                  int64_t kchunk) {
+  __shared__ __attribute__((aligned(16))) float sA[128][F32_LDK];
+  __shared__ __attribute__((aligned(16))) float sB[128][F32_LDK];
+  const TileIdx ti = map_tile(blockIdx.x, gridDim.x, tiles_m, tiles_n, d.B);
+  const int64_t m0 = ti.tm * 128, n0 = ti.tn * 64;
+  const int64_t kbeg = ti.ks * kchunk;
+  const int64_t kend = (kbeg + kchunk < d.K) ? kbeg + kchunk : d.K;
+  const c32* A = (const c32*)d.a + ti.batch * d.sab;
+  const c32* B = (const c32*)d.b + ti.batch * d.sbb;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = (w & 1) * 64, wn = (w >> 1) * 64;
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  c32 ra[4], rb[2];
+  load_c32_a<AK>(A, m0, kbeg, d.M, kend, d.sam, d.sak, ra);
+  load_c32_b<BK>(B, n0, kbeg, d.N, kend, d.sbn, d.sbk, rb);
+  for (int64_t k0 = kbeg; k0 < kend; k0 += C32_BK) {
+    {
+      const int r = AK ? (t >> 1) : (t & 127), kb = AK ? (t & 1) * 4 : (t >> 7) * 4;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        sA[r][kb + i] = ra[i].x;
+        sA[r][C32_BK + kb + i] = ra[i].y;
+      }
+    }
+    {
+      const int n = BK ? (t >> 2) : (t & 63), kb = BK ? (t & 3) * 2 : (t >> 6) * 2;
+      const int nre = (n & 31) + ((n >> 5) << 6), nim = nre + 32;
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        sB[nre][kb + i] = rb[i].x;
+        sB[nim][kb + i] = rb[i].y;
+        sB[nre][C32_BK + kb + i] = -rb[i].y;
+        sB[nim][C32_BK + kb + i] = rb[i].x;
+      }
+    }
+    __syncthreads();
+    if (k0 + C32_BK < kend) {
+      load_c32_a<AK>(A, m0, k0 + C32_BK, d.M, kend, d.sam, d.sak, ra);
+      load_c32_b<BK>(B, n0, k0 + C32_BK, d.N, kend, d.sbn, d.sbk, rb);
+    }
+    // lanes 0-31 hold k' = kb+s, lanes 32-63 hold k' = kb+4+s for the s-th MFMA (s < 4)
+#pragma unroll
+    for (int kb = 0; kb < 2 * C32_BK; kb += 8) {
+      const int kk = kb + 4 * (lane >> 5);
+      float4 af[2], bf[2];
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb) af[nb] = *(const float4*)&sB[wn + nb * 32 + (lane & 31)][kk];
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) bf[mb] = *(const float4*)&sA[wm + mb * 32 + (lane & 31)][kk];
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb) {
+          acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[nb].x, bf[mb].x, acc[nb][mb], 0, 0, 0);
+          acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[nb].y, bf[mb].y, acc[nb][mb], 0, 0, 0);
+          acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[nb].z, bf[mb].z, acc[nb][mb], 0, 0, 0);
+          acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[nb].w, bf[mb].w, acc[nb][mb], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  // lane holds m = m0 + wm + mb*32 + (lane & 31) and complex n = n0 + wn/2 + 8*(r/4) +
+  // 4*(lane/32) + r%4, with the real part in block nb = 0 and the imaginary part in nb = 1
+  const c32 sx = OpsC32::from(d.sx), sy = OpsC32::from(d.sy);
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int64_t m = m0 + wm + mb * 32 + (lane & 31);
+      const int64_t n = n0 + (wn >> 1) + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      if (m < d.M && n < d.N) {
+        const c32 v{acc[0][mb][r], acc[1][mb][r]};
+        if (d.ksplit > 1) {
+          ((c32*)d.work)[((ti.ks * d.B + ti.batch) * d.M) * d.N + m + n * d.M] = v;
+        } else {
+          c32* yp = (c32*)d.y + ti.batch * d.syb + m * d.sym + n * d.syn;
+          *yp = epilogue<OpsC32>(d.mode, sx, sy, v, yp);
+        }
+      }
+    }
+}
+
 // ---- FP64 ------------------------------------------------------------------------------------
 constexpr int F64_BK = 16;
 constexpr int F64_LDK = F64_BK + 2;
@@ -536,6 +669,51 @@ void launch_gemm_mfma(int prec, GemmDesc d, int ksplit, hipStream_t st) {
     if (g > 65536) g = 65536;
     if (f64) hipLaunchKernelGGL(k_splitk_reduce<OpsF64>, dim3(g), dim3(256), 0, st, d);
     else hipLaunchKernelGGL(k_splitk_reduce<OpsF32>, dim3(g), dim3(256), 0, st, d);
+  }
+}
+
+int mfma_c32_plan_ksplit(const GemmDesc& d) {
+  const int64_t tiles = ((d.M + 127) / 128) * ((d.N + 63) / 64) * d.B;
+  int ks = 1;
+  // as for the real kernels: chunks of at least 32 complex k (64 real) until ~512 workgroups
+  while (tiles * ks < 512 && d.K / (ks * 2) >= 32) ks *= 2;
+  return ks;
+}
+
+int64_t mfma_c32_workspace(const GemmDesc& d, int ksplit) {
+  if (ksplit <= 1) return 0;
+  return (int64_t)ksplit * d.B * d.M * d.N * (int64_t)sizeof(c32);
+}
+
+void launch_gemm_mfma_c32(GemmDesc d, int ksplit, hipStream_t st) {
+  if (d.M <= 0 || d.N <= 0 || d.B <= 0) return;
+  OME_DRY_RETURN(ksplit > 1 ? 2 : 1);
+  d.mode = epi_mode<OpsC32>(d.sx, d.sy);
+  if (ksplit < 1) ksplit = 1;
+  int64_t kchunk = (d.K + ksplit - 1) / ksplit;
+  kchunk = ((kchunk + C32_BK - 1) / C32_BK) * C32_BK;
+  if (kchunk < C32_BK) kchunk = C32_BK;
+  ksplit = (int)((d.K + kchunk - 1) / kchunk);
+  if (ksplit < 1) ksplit = 1;
+  d.ksplit = ksplit;
+  const int64_t tiles_m = (d.M + 127) / 128, tiles_n = (d.N + 63) / 64;
+  const int64_t blocks = tiles_m * tiles_n * d.B * ksplit;
+  const bool ak = (d.sak == 1) || (d.M == 1 && d.sam != 1);
+  const bool bk = (d.sbk == 1) || (d.N == 1 && d.sbn != 1);
+  const dim3 grid((uint32_t)blocks), block(256);
+  {
+    TimedScope ts(KC_GEMM_MFMA, st);
+    if (ak && bk) hipLaunchKernelGGL((k_mfma_c32<true, true>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk);
+    else if (ak) hipLaunchKernelGGL((k_mfma_c32<true, false>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk);
+    else if (bk) hipLaunchKernelGGL((k_mfma_c32<false, true>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk);
+    else hipLaunchKernelGGL((k_mfma_c32<false, false>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk);
+  }
+  if (ksplit > 1) {
+    TimedScope ts(KC_OTHER, st);
+    const int64_t outputs = d.M * d.N * d.B;
+    int g = (int)((outputs + 255) / 256);
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_splitk_reduce<OpsC32>, dim3(g), dim3(256), 0, st, d);
   }
 }
 

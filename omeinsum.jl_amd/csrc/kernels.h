@@ -87,6 +87,7 @@ constexpr int SF_MAXK = 8;       // contracted labels
 constexpr int SF_MAXNK = 64;     // contracted points
 constexpr int SF_MAXOUT = 8192;  // output points
 constexpr int SF_MAXLEAF = 256;  // leaves of a tree whose steps may be fused
+constexpr int SF_THREADS = 1024; // threads of the workgroup that runs one subtree
 struct FusedStep {
   int32_t a_leaf, b_leaf;        // >= 0: leaf index; < 0: the operand is in the arena
   int64_t a_off, b_off, y_off;   // arena byte offsets
@@ -195,6 +196,11 @@ void launch_grouped_permute(int dt, const GPDesc& g, int64_t split, hipStream_t 
 int mfma_plan_ksplit(const GemmDesc& d, int requested);
 int64_t mfma_workspace(int dt, const GemmDesc& d, int ksplit);
 void launch_gemm_mfma(int prec, GemmDesc d, int ksplit, hipStream_t st);
+// ComplexF32 at FP32 on the matrix cores, realified (4M) while staged; A needs a unit stride
+// along m or k and B along k or n (complex elements).
+int mfma_c32_plan_ksplit(const GemmDesc& d);
+int64_t mfma_c32_workspace(const GemmDesc& d, int ksplit);
+void launch_gemm_mfma_c32(GemmDesc d, int ksplit, hipStream_t st);
 
 // Packed BF16x3 GEMM (Float32, or ComplexF32 realified into one real GEMM).
 int64_t packed_bytes_a(int64_t M, int64_t N, int64_t K, int64_t batch);

@@ -309,7 +309,26 @@ Status run_gemm(ome_ctx* ctx, int dt, GemmDesc d, hipStream_t st, int precision)
     const bool cx = (dt == C32 || dt == C64);
     const int algo = ctx->opt.complex_algo ? RA_3M : RA_4M;
     const int64_t rm = d.M, rn = algo == RA_4M ? 2 * d.N : d.N, rk = algo == RA_4M ? 2 * d.K : d.K;
-    if (cx && fits && rm >= 32 && rn >= 32 && rk >= 16 && d.M * d.N * d.K * d.B >= (1ll << 16))
+    const bool big = fits && rm >= 32 && rn >= 32 && rk >= 16 && d.M * d.N * d.K * d.B >= (1ll << 16);
+    // ComplexF32 in the four-real-products form with unit-stride operands: one kernel builds
+    // the real operands while staging them and writes the complex result itself
+    if (dt == C32 && precision != PREC_FP64 && algo == RA_4M && big &&
+        (d.sam == 1 || d.sak == 1 || d.M == 1 || d.K == 1) &&
+        (d.sbk == 1 || d.sbn == 1 || d.N == 1 || d.K == 1)) {
+      if (d.M == 1 && d.sak != 1) d.sam = 1;
+      if (d.K == 1) { if (d.sam != 1) d.sak = 1; if (d.sbn != 1) d.sbk = 1; }
+      if (d.N == 1 && d.sbk != 1) d.sbn = 1;
+      const int ks = mfma_c32_plan_ksplit(d);
+      Temp ws;
+      if (ks > 1) {
+        ws = Temp(ctx, (size_t)mfma_c32_workspace(d, ks));
+        if (!ws.p) return Status::err(OME_ERR_OOM, "device pool exhausted (split-K workspace)");
+        d.work = ws.p;
+      }
+      launch_gemm_mfma_c32(d, ks, st);
+      return Status::ok();
+    }
+    if (cx && big)
       return run_gemm_via_real(ctx, dt, d, st, precision, algo);
     if (dt == C32 && precision == PREC_FP64 && fits)
       return run_gemm_via_real(ctx, dt, d, st, precision, algo);

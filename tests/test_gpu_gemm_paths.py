@@ -56,7 +56,8 @@ def oracle(ixs, iy, xs, sd, y0=None, sx=True, sy=False, tropical=False):
                             sy=sy, tropical=tropical)
 
 
-SPECS = ["ij,jk->ik", "ji,kj->ik", "ijb,jkb->ikb", "ajb,bkj->bka", "ij,jk->ki"]
+SPECS = ["ij,jk->ik", "ji,kj->ik", "ji,jk->ik", "ij,kj->ik", "ijb,jkb->ikb", "ajb,bkj->bka",
+         "ij,jk->ki"]
 SIZES = {"i": 70, "j": 90, "k": 65, "a": 33, "b": 3}
 
 
@@ -80,6 +81,25 @@ def test_complex_as_real_gemm(spec, algo, dtype, prec):
     got = run(ixs, iy, xs, SIZES, be, y0=y0, sx=sx, sy=sy)
     ref = oracle(ixs, iy, xs, SIZES, y0=y0, sx=sx, sy=sy)
     assert_close(got, ref, dtype, 16 * k * (3 if algo == "3m" else 1), spec + " scaled")
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_complex_split_k_and_edges(prec):
+    # one 128 x 64 output tile and a long K: the ComplexF32 matrix-core kernel splits K and
+    # sums the partials in a second pass; odd sizes leave partial tiles and a partial K-step
+    be = om().MI355XBackend(precision=prec)
+    r = rng(2102)
+    for spec, sd in [("ij,jk->ik", {"i": 40, "j": 3001, "k": 24}),
+                     ("ji,kj->ki", {"i": 129, "j": 77, "k": 65}),
+                     ("ijb,jkb->ikb", {"i": 33, "j": 250, "k": 17, "b": 5})]:
+        ixs, iy = parse(spec)
+        xs = [rand(r, [sd[l] for l in ix], np.complex64) for ix in ixs]
+        got = run(ixs, iy, xs, sd, be)
+        assert_close(got, oracle(ixs, iy, xs, sd), np.complex64, 4 * sd["j"], spec)
+        y0 = rand(r, [sd[l] for l in iy], np.complex64)
+        got = run(ixs, iy, xs, sd, be, y0=y0, sx=0.5 + 2j, sy=-1 + 0.5j)
+        ref = oracle(ixs, iy, xs, sd, y0=y0, sx=0.5 + 2j, sy=-1 + 0.5j)
+        assert_close(got, ref, np.complex64, 4 * sd["j"] + 8, spec + " scaled")
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.complex64])
