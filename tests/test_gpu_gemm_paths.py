@@ -102,6 +102,42 @@ def test_complex_split_k_and_edges(prec):
         assert_close(got, ref, np.complex64, 4 * sd["j"] + 8, spec + " scaled")
 
 
+def scattered(r, m_sizes, k_sizes, n_sizes):
+    """A binary code whose M, K and N labels are interleaved at random in both operands and in
+    the output (no operand is a GEMM-ready run of groups)."""
+    ml = list(range(len(m_sizes)))
+    kl = [100 + q for q in range(len(k_sizes))]
+    nl = [200 + q for q in range(len(n_sizes))]
+    sd = dict(zip(ml, m_sizes)) | dict(zip(kl, k_sizes)) | dict(zip(nl, n_sizes))
+    a = [int(x) for x in r.permutation(ml + kl)]
+    b = [int(x) for x in r.permutation(kl + nl)]
+    y = [int(x) for x in r.permutation(ml + nl)]
+    return [a, b], y, sd
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_complex_scattered_labels(prec):
+    # ComplexF32 products whose labels are scattered over both operands and the output (the
+    # tensor-network case): network-like size-2 labels, mixed sizes, a long K (split), a short
+    # K (other routes), and scaled accumulation into y
+    be = om().MI355XBackend(precision=prec)
+    r = rng(2103)
+    cases = [([2] * 6, [2] * 5, [2] * 5),          # M 64, K 32, N 32
+             ([8, 9], [3, 4, 5], [5, 6]),           # M 72, K 60, N 30
+             ([2] * 5, [2] * 12, [2] * 4),          # M 32, K 4096, N 16: split K
+             ([3, 5, 7], [2, 3], [4, 4, 2])]        # M 105, K 6
+    for ms, ks, ns in cases:
+        ixs, iy, sd = scattered(r, ms, ks, ns)
+        xs = [rand(r, [sd[l] for l in ix], np.complex64) for ix in ixs]
+        k = int(np.prod(ks))
+        got = run(ixs, iy, xs, sd, be)
+        assert_close(got, oracle(ixs, iy, xs, sd), np.complex64, 4 * k, str((ms, ks, ns)))
+        y0 = rand(r, [sd[l] for l in iy], np.complex64)
+        got = run(ixs, iy, xs, sd, be, y0=y0, sx=0.5 - 1j, sy=2 + 0.25j)
+        ref = oracle(ixs, iy, xs, sd, y0=y0, sx=0.5 - 1j, sy=2 + 0.25j)
+        assert_close(got, ref, np.complex64, 4 * k + 8, str((ms, ks, ns)) + " scaled")
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.complex64])
 def test_fp64_precision_rounds_once(dtype):
     # precision="fp64": Float32 / ComplexF32 operands are widened and multiplied on the FP64
