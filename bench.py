@@ -631,10 +631,10 @@ def main():
         sys.exit(1)
     import omeinsum_mi355x as om  # noqa: F401
 
-    precs = ("bf16x3", "fp32") if args.workload == "all" else ("bf16x3",)
+    precs = ("bf16x3", "fp32") if args.workload in ("all", "c4") else ("bf16x3",)
     c4, ab = bench_c4(args.steps, args.warmup, precs)
     extra = {"c4": c4}
-    if args.workload == "all":
+    if args.workload in ("all", "c4"):
         extra["c4_more"] = bench_c4_more(ab, args.steps, args.warmup)
     if args.workload in ("all", "c1"):
         extra["c1_ijk_jkl_il_64"] = bench_c1(args.steps, args.warmup)

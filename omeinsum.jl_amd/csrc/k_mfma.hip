@@ -331,10 +331,12 @@ __device__ __forceinline__ void load_c32_a(const c32* __restrict__ P, int64_t r0
   const int t = threadIdx.x;
   const int64_t r = r0 + (KMAJ ? (t >> 1) : (t & 127));
   const int64_t kb = k0 + (KMAJ ? (t & 1) * 4 : (t >> 7) * 4);
+  // every lane loads (outside the tile: the first element), the zeros are applied when the
+  // values are staged, so no wait follows the loads here
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const bool ok = r < R && kb + i < K;
-    v[i] = ok ? P[r * sr + (kb + i) * sk] : c32{0.f, 0.f};
+    v[i] = P[ok ? r * sr + (kb + i) * sk : 0];
   }
 }
 
@@ -350,7 +352,7 @@ __device__ __forceinline__ void load_c32_b(const c32* __restrict__ P, int64_t r0
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
     const bool ok = r < R && kb + i < K;
-    v[i] = ok ? P[r * sr + (kb + i) * sk] : c32{0.f, 0.f};
+    v[i] = P[ok ? r * sr + (kb + i) * sk : 0];
   }
 }
 
@@ -379,22 +381,28 @@ __global__ __launch_bounds__(256) void k_mfma_c32(GemmDesc d, int64_t tiles_m, i
   load_c32_b<BK>(B, n0, kbeg, d.N, kend, d.sbn, d.sbk, rb);
   for (int64_t k0 = kbeg; k0 < kend; k0 += C32_BK) {
     {
+      // elements outside the operand (loaded from its first element) stage as zeros
       const int r = AK ? (t >> 1) : (t & 127), kb = AK ? (t & 1) * 4 : (t >> 7) * 4;
+      const bool rok = m0 + r < d.M;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        sA[r][kb + i] = ra[i].x;
-        sA[r][C32_BK + kb + i] = ra[i].y;
+        const bool ok = rok && k0 + kb + i < kend;
+        sA[r][kb + i] = ok ? ra[i].x : 0.f;
+        sA[r][C32_BK + kb + i] = ok ? ra[i].y : 0.f;
       }
     }
     {
       const int n = BK ? (t >> 2) : (t & 63), kb = BK ? (t & 3) * 2 : (t >> 6) * 2;
       const int nre = (n & 31) + ((n >> 5) << 6), nim = nre + 32;
+      const bool nok = n0 + n < d.N;
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        sB[nre][kb + i] = rb[i].x;
-        sB[nim][kb + i] = rb[i].y;
-        sB[nre][C32_BK + kb + i] = -rb[i].y;
-        sB[nim][C32_BK + kb + i] = rb[i].x;
+        const bool ok = nok && k0 + kb + i < kend;
+        const float br = ok ? rb[i].x : 0.f, bi = ok ? rb[i].y : 0.f;
+        sB[nre][kb + i] = br;
+        sB[nim][kb + i] = bi;
+        sB[nre][C32_BK + kb + i] = -bi;
+        sB[nim][C32_BK + kb + i] = br;
       }
     }
     __syncthreads();
