@@ -23,10 +23,92 @@ namespace ome {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
+// Loads and stores through global-address-space pointers: the problem descriptors hold generic
+// pointers, and generic (flat) accesses would also count against the LDS wait counter.
+typedef float gg_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gg_load4(const float* p) {
+  const gg_f4 v = *(const __attribute__((address_space(1))) gg_f4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float gg_load(const float* p) {
+  return *(const __attribute__((address_space(1))) float*)p;
+}
+__device__ __forceinline__ float* gg_global(float* p) {
+  return (float*)(__attribute__((address_space(1))) float*)p;
+}
+
+// One K-step of an operand tile as float4 groups q (A: 4 rows m..m+3 at one k; B: 4 k at one
+// n).  Every lane loads: elements outside the problem read the operand's first element and
+// their bit (4q + i) is left clear in the returned mask; gg_stage_zero applies the zeros when
+// the step is staged, so no wait follows the loads.
+template <int NQ>
+__device__ __forceinline__ uint32_t gg_load_a(const float* A, int64_t M, int64_t K, bool vec,
+                                              int64_t m0, int64_t k0, int TM, float4 (&ra)[NQ]) {
+  uint32_t mask = 0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int e = threadIdx.x + 256 * q;
+    const int64_t m = m0 + (e % (TM / 4)) * 4, k = k0 + e / (TM / 4);
+    if (vec) {  // M % 4 == 0: a float4 is wholly inside or wholly outside
+      const bool ok = m < M && k < K;
+      ra[q] = gg_load4(A + (ok ? m + k * M : 0));
+      mask |= ok ? 0xFu << (4 * q) : 0u;
+    } else {
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const bool ok = m + i < M && k < K;
+        v[i] = gg_load(A + (ok ? m + i + k * M : 0));
+        mask |= ok ? 1u << (4 * q + i) : 0u;
+      }
+      ra[q] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+  }
+  return mask;
+}
+template <int NQ>
+__device__ __forceinline__ uint32_t gg_load_b(const float* B, int64_t K, int64_t N, bool vec,
+                                              int64_t n0, int64_t k0, int BK, float4 (&rb)[NQ]) {
+  uint32_t mask = 0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int e = threadIdx.x + 256 * q;
+    const int64_t k = k0 + (e % (BK / 4)) * 4, n = n0 + e / (BK / 4);
+    if (vec) {  // K % 4 == 0
+      const bool ok = k < K && n < N;
+      rb[q] = gg_load4(B + (ok ? k + n * K : 0));
+      mask |= ok ? 0xFu << (4 * q) : 0u;
+    } else {
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const bool ok = k + i < K && n < N;
+        v[i] = gg_load(B + (ok ? k + i + n * K : 0));
+        mask |= ok ? 1u << (4 * q + i) : 0u;
+      }
+      rb[q] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+  }
+  return mask;
+}
+template <int NQ>
+__device__ __forceinline__ void gg_stage_zero(float4 (&r)[NQ], uint32_t mask) {
+  if (mask == (NQ == 8 ? 0xFFFFFFFFu : (1u << (4 * NQ)) - 1u)) return;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    r[q].x = (mask >> (4 * q)) & 1u ? r[q].x : 0.f;
+    r[q].y = (mask >> (4 * q + 1)) & 1u ? r[q].y : 0.f;
+    r[q].z = (mask >> (4 * q + 2)) & 1u ? r[q].z : 0.f;
+    r[q].w = (mask >> (4 * q + 3)) & 1u ? r[q].w : 0.f;
+  }
+}
+
 // ---- grouped GEMM, Float32 on the matrix cores -------------------------------------------
 
 template <int TM, int TN, int BK>
-__global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g) {
+__global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g, const GGProb* __restrict__ probs,
+                                                     const int64_t* __restrict__ first,
+                                                     const int32_t* __restrict__ map) {
   constexpr int LDK = BK + 4;
   constexpr int MB = TM / 64, NB = TN / 64;       // 32x32 blocks per wave along m / n
   constexpr int NA = TM * BK / 1024;              // float4 loads per thread, A tile
@@ -34,13 +116,15 @@ __global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g) {
   static_assert(NA >= 1 && NBL >= 1, "tile too small for 256 threads");
   __shared__ __attribute__((aligned(16))) float sA[TM][LDK];
   __shared__ __attribute__((aligned(16))) float sB[TN][LDK];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
   const float alpha = (float)g.alpha.re, beta = (float)g.beta.re;
   for (int64_t tile = blockIdx.x; tile < g.total_tiles; tile += gridDim.x) {
-    const int p = g.map[tile];
-    const GGProb pr = g.probs[p];
-    const int64_t r = tile - g.first[p];
+    // the tables are restrict arguments: the kernel's stores cannot clobber them, so they are
+    // read with scalar loads and every per-problem branch below is uniform
+    const int p = map[tile];
+    const GGProb pr = probs[p];
+    const int64_t r = tile - first[p];
     const int64_t per = (int64_t)pr.tiles_m * pr.tiles_n;
     const int64_t batch = r / per;
     const int64_t rr = r - batch * per;
@@ -58,38 +142,16 @@ __global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g) {
     // A tile (m fastest in memory): load q covers m4 = (e % (TM/4))*4, k = e / (TM/4)
     // B tile (k fastest in memory): load q covers k4 = (e % (BK/4))*4, n = e / (BK/4)
     float4 ra[NA], rb[NBL];
+    uint32_t mka = 0, mkb = 0;
     auto load = [&](int64_t k0) __attribute__((always_inline)) {
-#pragma unroll
-      for (int q = 0; q < NA; ++q) {
-        const int e = t + 256 * q;
-        const int64_t m = m0 + (e % (TM / 4)) * 4, k = k0 + e / (TM / 4);
-        const float* src = A + m + k * M;
-        if (pr.vec_a && m + 4 <= M && k < K) {
-          ra[q] = *(const float4*)src;
-        } else {
-          ra[q].x = (m < M && k < K) ? src[0] : 0.f;
-          ra[q].y = (m + 1 < M && k < K) ? src[1] : 0.f;
-          ra[q].z = (m + 2 < M && k < K) ? src[2] : 0.f;
-          ra[q].w = (m + 3 < M && k < K) ? src[3] : 0.f;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < NBL; ++q) {
-        const int e = t + 256 * q;
-        const int64_t k = k0 + (e % (BK / 4)) * 4, n = n0 + e / (BK / 4);
-        const float* src = B + k + n * K;
-        if (pr.vec_b && k + 4 <= K && n < N) {
-          rb[q] = *(const float4*)src;
-        } else {
-          rb[q].x = (k < K && n < N) ? src[0] : 0.f;
-          rb[q].y = (k + 1 < K && n < N) ? src[1] : 0.f;
-          rb[q].z = (k + 2 < K && n < N) ? src[2] : 0.f;
-          rb[q].w = (k + 3 < K && n < N) ? src[3] : 0.f;
-        }
-      }
+      mka = gg_load_a<NA>(A, M, K, pr.vec_a, m0, k0, TM, ra);
+      mkb = gg_load_b<NBL>(B, K, N, pr.vec_b, n0, k0, BK, rb);
     };
-    load(0);
+    // an empty sum (K == 0) loads nothing: its operands may hold no element at all
+    if (K > 0) load(0);
     for (int64_t k0 = 0; k0 < K; k0 += BK) {
+      gg_stage_zero<NA>(ra, mka);
+      gg_stage_zero<NBL>(rb, mkb);
 #pragma unroll
       for (int q = 0; q < NA; ++q) {
         const int e = t + 256 * q;
@@ -142,7 +204,7 @@ __global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g) {
           const int64_t m = m0 + wm + mb * 32 + (lane & 31);
           const int64_t n = n0 + wn + nb * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
           if (m < M && n < N) {
-            float* yp = C + m + n * M;
+            float* yp = gg_global(C + m + n * M);
             *yp = epilogue<OpsF32>(g.mode, alpha, beta, acc[nb][mb][q], yp);
           }
         }
@@ -168,19 +230,23 @@ __device__ __forceinline__ void gg_split(float x, uint16_t& hi, uint16_t& lo) {
 }
 
 template <int TM, int TN>
-__global__ __launch_bounds__(256) void k_grouped_bf16x3(GGDesc g) {
+__global__ __launch_bounds__(256) void k_grouped_bf16x3(GGDesc g, const GGProb* __restrict__ probs,
+                                                        const int64_t* __restrict__ first,
+                                                        const int32_t* __restrict__ map) {
   constexpr int BK = 32, LDK = 40;
   constexpr int MB = TM / 64, NB = TN / 64;
   constexpr int NA = TM * BK / 1024, NBL = TN * BK / 1024;
   __shared__ __attribute__((aligned(16))) uint16_t sA[2][TM][LDK];  // [hi/lo][m][k]
   __shared__ __attribute__((aligned(16))) uint16_t sB[2][TN][LDK];  // [hi/lo][n][k]
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
   const float alpha = (float)g.alpha.re, beta = (float)g.beta.re;
   for (int64_t tile = blockIdx.x; tile < g.total_tiles; tile += gridDim.x) {
-    const int p = g.map[tile];
-    const GGProb pr = g.probs[p];
-    const int64_t r = tile - g.first[p];
+    // the tables are restrict arguments: the kernel's stores cannot clobber them, so they are
+    // read with scalar loads and every per-problem branch below is uniform
+    const int p = map[tile];
+    const GGProb pr = probs[p];
+    const int64_t r = tile - first[p];
     const int64_t per = (int64_t)pr.tiles_m * pr.tiles_n;
     const int64_t batch = r / per;
     const int64_t rr = r - batch * per;
@@ -196,38 +262,16 @@ __global__ __launch_bounds__(256) void k_grouped_bf16x3(GGDesc g) {
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.f;
     float4 ra[NA], rb[NBL];
+    uint32_t mka = 0, mkb = 0;
     auto load = [&](int64_t k0) __attribute__((always_inline)) {
-#pragma unroll
-      for (int q = 0; q < NA; ++q) {
-        const int e = t + 256 * q;
-        const int64_t m = m0 + (e % (TM / 4)) * 4, k = k0 + e / (TM / 4);
-        const float* src = A + m + k * M;
-        if (pr.vec_a && m + 4 <= M && k < K) {
-          ra[q] = *(const float4*)src;
-        } else {
-          ra[q].x = (m < M && k < K) ? src[0] : 0.f;
-          ra[q].y = (m + 1 < M && k < K) ? src[1] : 0.f;
-          ra[q].z = (m + 2 < M && k < K) ? src[2] : 0.f;
-          ra[q].w = (m + 3 < M && k < K) ? src[3] : 0.f;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < NBL; ++q) {
-        const int e = t + 256 * q;
-        const int64_t k = k0 + (e % (BK / 4)) * 4, n = n0 + e / (BK / 4);
-        const float* src = B + k + n * K;
-        if (pr.vec_b && k + 4 <= K && n < N) {
-          rb[q] = *(const float4*)src;
-        } else {
-          rb[q].x = (k < K && n < N) ? src[0] : 0.f;
-          rb[q].y = (k + 1 < K && n < N) ? src[1] : 0.f;
-          rb[q].z = (k + 2 < K && n < N) ? src[2] : 0.f;
-          rb[q].w = (k + 3 < K && n < N) ? src[3] : 0.f;
-        }
-      }
+      mka = gg_load_a<NA>(A, M, K, pr.vec_a, m0, k0, TM, ra);
+      mkb = gg_load_b<NBL>(B, K, N, pr.vec_b, n0, k0, BK, rb);
     };
-    load(0);
+    // an empty sum (K == 0) loads nothing: its operands may hold no element at all
+    if (K > 0) load(0);
     for (int64_t k0 = 0; k0 < K; k0 += BK) {
+      gg_stage_zero<NA>(ra, mka);
+      gg_stage_zero<NBL>(rb, mkb);
 #pragma unroll
       for (int q = 0; q < NA; ++q) {
         const int e = t + 256 * q;
@@ -296,7 +340,7 @@ __global__ __launch_bounds__(256) void k_grouped_bf16x3(GGDesc g) {
           const int64_t m = m0 + wm + mb * 32 + (lane & 31);
           const int64_t n = n0 + wn + nb * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
           if (m < M && n < N) {
-            float* yp = C + m + n * M;
+            float* yp = gg_global(C + m + n * M);
             *yp = epilogue<OpsF32>(g.mode, alpha, beta, acc[nb][mb][q], yp);
           }
         }
@@ -564,15 +608,15 @@ void launch_grouped_gemm(int dt, int variant, const GGDesc& g, int grid, hipStre
     else if (variant == GG_BF16X3_W32)
       hipLaunchKernelGGL((k_grouped_bf16x3_wave<2>), dim3(grid), dim3(256), 0, st, g);
     else if (variant == GG_BF16X3_64)
-      hipLaunchKernelGGL((k_grouped_bf16x3<64, 64>), dim3(grid), dim3(256), 0, st, g);
+      hipLaunchKernelGGL((k_grouped_bf16x3<64, 64>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     else if (variant == GG_BF16X3_128)
-      hipLaunchKernelGGL((k_grouped_bf16x3<128, 128>), dim3(grid), dim3(256), 0, st, g);
+      hipLaunchKernelGGL((k_grouped_bf16x3<128, 128>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     else if (variant == GG_F32_128)
-      hipLaunchKernelGGL((k_grouped_f32<128, 128, 16>), dim3(grid), dim3(256), 0, st, g);
+      hipLaunchKernelGGL((k_grouped_f32<128, 128, 16>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     else if (variant == GG_F32_64_BK32)
-      hipLaunchKernelGGL((k_grouped_f32<64, 64, 32>), dim3(grid), dim3(256), 0, st, g);
+      hipLaunchKernelGGL((k_grouped_f32<64, 64, 32>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     else
-      hipLaunchKernelGGL((k_grouped_f32<64, 64, 16>), dim3(grid), dim3(256), 0, st, g);
+      hipLaunchKernelGGL((k_grouped_f32<64, 64, 16>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     return;
   }
   OME_DISPATCH(dt, O, {
