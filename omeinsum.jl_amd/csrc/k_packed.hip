@@ -49,23 +49,27 @@ struct PackDesc {
   int rfast;          // 1: unit stride along rows (read rows-fastest)
 };
 
-__device__ __forceinline__ float pack_fetch(const PackDesc& d, const float* __restrict__ base,
-                                            int64_t r, int64_t k) {
-  if (r >= d.R || k >= d.K) return 0.f;
-  if (d.mode == 0) return base[r * d.sr + k * d.sk];
+// The element (r, k) of the packed operand as a float address (the operand's first float
+// when (r, k) lies outside it) and whether its value is negated or replaced by zero: the
+// general pack paths issue a batch of these loads before any of them is used.
+__device__ __forceinline__ const float* pack_addr(const PackDesc& d, const float* __restrict__ base,
+                                                  int64_t r, int64_t k, bool& neg, bool& ok) {
+  ok = r < d.R && k < d.K;
+  neg = false;
+  if (!ok) return base;
+  if (d.mode == 0) return base + r * d.sr + k * d.sk;
   if (d.mode == 1) {  // A' = [Ar | Ai]
     const bool im = k >= d.K0;
     const int64_t kk = im ? k - d.K0 : k;
-    return base[2 * (r * d.sr + kk * d.sk) + (im ? 1 : 0)];
+    return base + 2 * (r * d.sr + kk * d.sk) + (im ? 1 : 0);
   }
   // B' (2K x 2N) stored as rows n' (2N), k' (2K): B'[k'][n'] with
   //   k' <  K, n' <  N:  Br[k][n]     k' <  K, n' >= N:  Bi[k][n]
   //   k' >= K, n' <  N: -Bi[k][n]     k' >= K, n' >= N:  Br[k][n]
   const bool kim = k >= d.K0, nim = r >= d.R0;
   const int64_t kk = kim ? k - d.K0 : k, nn = nim ? r - d.R0 : r;
-  const float* p = base + 2 * (nn * d.sr + kk * d.sk);
-  if (!kim) return nim ? p[1] : p[0];
-  return nim ? p[0] : -p[1];
+  neg = kim && !nim;
+  return base + 2 * (nn * d.sr + kk * d.sk) + (kim != nim ? 1 : 0);
 }
 
 // TR rows per packed tile (128 or 256).
@@ -115,20 +119,30 @@ __device__ __forceinline__ void pack_tile(const PackDesc& d, int64_t bid, float 
       tile[r][k4 + 2] = v[i].z;
       tile[r][k4 + 3] = v[i].w;
     }
-  } else if (d.rfast) {
-    // lanes along rows
-#pragma unroll 4
-    for (int i = 0; i < TR / 8; ++i) {
-      const int e = t + 256 * i;
-      const int r = e % TR, k = e / TR;
-      tile[r][k] = pack_fetch(d, base, r0 + r, k0 + k);
-    }
   } else {
-    // lanes along k
-#pragma unroll 4
-    for (int i = 0; i < TR / 8; ++i) {
-      const int k = t & 31, r = (t >> 5) + 8 * i;
-      tile[r][k] = pack_fetch(d, base, r0 + r, k0 + k);
+    // lanes along rows (rfast) or along k; batches of 8 loads issued before their LDS stores
+#pragma unroll
+    for (int i0 = 0; i0 < TR / 8; i0 += 8) {
+      float v[8];
+      uint32_t ok = 0, neg = 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int i = i0 + j;
+        const int r = d.rfast ? (t + 256 * i) % TR : (t >> 5) + 8 * i;
+        const int k = d.rfast ? (t + 256 * i) / TR : (t & 31);
+        bool n_, o_;
+        v[j] = *pack_addr(d, base, r0 + r, k0 + k, n_, o_);
+        ok |= o_ ? 1u << j : 0u;
+        neg |= n_ ? 1u << j : 0u;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int i = i0 + j;
+        const int r = d.rfast ? (t + 256 * i) % TR : (t >> 5) + 8 * i;
+        const int k = d.rfast ? (t + 256 * i) / TR : (t & 31);
+        const float x = (neg >> j) & 1u ? -v[j] : v[j];
+        tile[r][k] = (ok >> j) & 1u ? x : 0.f;
+      }
     }
   }
   __syncthreads();
