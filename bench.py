@@ -53,7 +53,9 @@ def log(*a):
 
 def time_events(fn, steps, warmup, after_warmup=None):
     """Average milliseconds per call: HIP events on torch's current stream (the library
-    launches on that stream) bracketing `steps` calls after `warmup` calls."""
+    launches on that stream) bracketing `steps` calls after `warmup` calls.  Python's garbage
+    collector is paused over the timed calls, as timeit does."""
+    import gc
     import torch
     for _ in range(warmup):
         fn()
@@ -62,11 +64,17 @@ def time_events(fn, steps, warmup, after_warmup=None):
         after_warmup()
     s = torch.cuda.Event(enable_timing=True)
     e = torch.cuda.Event(enable_timing=True)
-    s.record()
-    for _ in range(steps):
-        fn()
-    e.record()
-    torch.cuda.synchronize()
+    gc_was = gc.isenabled()
+    gc.disable()
+    try:
+        s.record()
+        for _ in range(steps):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+    finally:
+        if gc_was:
+            gc.enable()
     return s.elapsed_time(e) / steps
 
 
@@ -222,7 +230,11 @@ def bench_c1(steps, warmup):
             b = torch.rand(64, 64, 64, dtype=dt, device="cuda") * 2 - 1
         sd = om.get_size_dict(code.ixs, (a, b))
         y = torch.empty(64, 64, dtype=dt, device="cuda")
-        ms = time_events(lambda: om.einsum_inplace(code, (a, b), y, True, False, sd), steps * 5, warmup)
+        # host-latency bound: the median of five timed runs, so that one stall of the host
+        # (outside the library) does not stand for the call
+        ms = statistics.median(
+            time_events(lambda: om.einsum_inplace(code, (a, b), y, True, False, sd), steps, warmup)
+            for _ in range(5))
         esz = 4 if dt == torch.float32 else 8
         nbytes = esz * (2 * 64 ** 3 + 64 * 64)
         out[name] = {"us": ms * 1e3, "tflops": 2 * 64 ** 4 / ms / 1e9, "gbs": nbytes / ms / 1e6,

@@ -249,13 +249,18 @@ __global__ __launch_bounds__(256) void k_permute(PermDesc d) {
       rem = q;
     }
     const int64_t ybase = (int64_t)row * d0;
+    // the run's loads are all issued before its stores (elements past the row load the run's
+    // first element, which is in the row)
+    T v[ELEMS];
 #pragma unroll
     for (int e = 0; e < ELEMS; ++e) {
-      uint32_t i = i0 + e;
-      if (i < d0) {
-        T v = conj_if<O>(x[off + (int64_t)i * s0], d.conj);
-        y[ybase + i] = epilogue<O>(d.mode, sx, sy, v, y + ybase + i);
-      }
+      const uint32_t i = i0 + e;
+      v[e] = x[off + (int64_t)(i < d0 ? i : i0) * s0];
+    }
+#pragma unroll
+    for (int e = 0; e < ELEMS; ++e) {
+      const uint32_t i = i0 + e;
+      if (i < d0) y[ybase + i] = epilogue<O>(d.mode, sx, sy, conj_if<O>(v[e], d.conj), y + ybase + i);
     }
   }
 }

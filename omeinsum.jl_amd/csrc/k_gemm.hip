@@ -41,17 +41,30 @@ __global__ __launch_bounds__(256) void k_gemm_tile(GemmDesc d, int64_t tiles_m, 
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = O::zero();
   for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+    // all eight loads of the step are issued before any is staged: lanes outside the operand
+    // load its first element and stage a zero
+    T av[4], bv[4];
+    uint32_t ok = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      int e = t + 256 * q;
-      int mm, kk;
-      if (a_mfast) { mm = e & 63; kk = e >> 6; } else { kk = e & 15; mm = e >> 4; }
-      int64_t gm = m0 + mm, gk = k0 + kk;
-      As[kk][mm] = (gm < d.M && gk < kend) ? Ab[gm * d.sam + gk * d.sak] : O::zero();
-      int nn, kb;
-      if (b_kfast) { kb = e & 15; nn = e >> 4; } else { nn = e & 63; kb = e >> 6; }
-      int64_t gn = n0 + nn, gkb = k0 + kb;
-      Bs[kb][nn] = (gn < d.N && gkb < kend) ? Bb[gkb * d.sbk + gn * d.sbn] : O::zero();
+      const int e = t + 256 * q;
+      const int mm = a_mfast ? (e & 63) : (e >> 4), kk = a_mfast ? (e >> 6) : (e & 15);
+      const int64_t gm = m0 + mm, gk = k0 + kk;
+      const bool oka = gm < d.M && gk < kend;
+      av[q] = Ab[oka ? gm * d.sam + gk * d.sak : 0];
+      const int nn = b_kfast ? (e >> 4) : (e & 63), kb = b_kfast ? (e & 15) : (e >> 6);
+      const int64_t gn = n0 + nn, gkb = k0 + kb;
+      const bool okb = gn < d.N && gkb < kend;
+      bv[q] = Bb[okb ? gkb * d.sbk + gn * d.sbn : 0];
+      ok |= (oka ? 1u : 0u) << q | (okb ? 16u : 0u) << q;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = t + 256 * q;
+      const int mm = a_mfast ? (e & 63) : (e >> 4), kk = a_mfast ? (e >> 6) : (e & 15);
+      As[kk][mm] = (ok >> q) & 1u ? av[q] : O::zero();
+      const int nn = b_kfast ? (e >> 4) : (e & 63), kb = b_kfast ? (e & 15) : (e >> 6);
+      Bs[kb][nn] = (ok >> (4 + q)) & 1u ? bv[q] : O::zero();
     }
     __syncthreads();
     const int kn = (int)((kend - k0) < BK ? (kend - k0) : BK);
