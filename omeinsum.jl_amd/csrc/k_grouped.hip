@@ -528,7 +528,9 @@ __global__ __launch_bounds__(256) void k_grouped_bf16x3_wave(GGDesc g) {
 // ---- grouped GEMM, semiring VALU tile (any element type) --------------------------------
 
 template <class O>
-__global__ __launch_bounds__(256) void k_grouped_valu(GGDesc g) {
+__global__ __launch_bounds__(256) void k_grouped_valu(GGDesc g, const GGProb* __restrict__ probs,
+                                                      const int64_t* __restrict__ first,
+                                                      const int32_t* __restrict__ map) {
   using T = typename O::T;
   constexpr int BM = 64, BN = 64, BK = 16;
   __shared__ T As[BK][BM + 1];
@@ -536,9 +538,9 @@ __global__ __launch_bounds__(256) void k_grouped_valu(GGDesc g) {
   const int t = threadIdx.x, tm = t & 15, tn = t >> 4;
   const T alpha = O::from(g.alpha), beta = O::from(g.beta);
   for (int64_t tile = blockIdx.x; tile < g.total_tiles; tile += gridDim.x) {
-    const int p = g.map[tile];
-    const GGProb pr = g.probs[p];
-    const int64_t r = tile - g.first[p];
+    const int p = map[tile];
+    const GGProb pr = probs[p];
+    const int64_t r = tile - first[p];
     const int64_t per = (int64_t)pr.tiles_m * pr.tiles_n;
     const int64_t batch = r / per;
     const int64_t rr = r - batch * per;
@@ -552,15 +554,26 @@ __global__ __launch_bounds__(256) void k_grouped_valu(GGDesc g) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = O::zero();
     for (int64_t k0 = 0; k0 < K; k0 += BK) {
+      // the step's eight loads are issued before any is staged (lanes outside the problem load
+      // its first element and stage a zero)
+      T av[4], bv[4];
+      uint32_t ok = 0;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int e = t + 256 * q;
-        const int mm = e & 63, kk = e >> 6;  // A: m fastest
-        const int64_t gm = m0 + mm, gk = k0 + kk;
-        As[kk][mm] = (gm < M && gk < K) ? A[gm + gk * M] : O::zero();
-        const int kb = e & 15, nn = e >> 4;  // B: k fastest
-        const int64_t gn = n0 + nn, gkb = k0 + kb;
-        Bs[kb][nn] = (gn < N && gkb < K) ? B[gkb + gn * K] : O::zero();
+        const int64_t gm = m0 + (e & 63), gk = k0 + (e >> 6);  // A: m fastest
+        const bool oka = gm < M && gk < K;
+        av[q] = A[oka ? gm + gk * M : 0];
+        const int64_t gn = n0 + (e >> 4), gkb = k0 + (e & 15);  // B: k fastest
+        const bool okb = gn < N && gkb < K;
+        bv[q] = B[okb ? gkb + gn * K : 0];
+        ok |= (oka ? 1u : 0u) << q | (okb ? 16u : 0u) << q;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = t + 256 * q;
+        As[e >> 6][e & 63] = (ok >> q) & 1u ? av[q] : O::zero();
+        Bs[e & 15][e >> 4] = (ok >> (4 + q)) & 1u ? bv[q] : O::zero();
       }
       __syncthreads();
       const int kn = (int)((K - k0) < BK ? (K - k0) : BK);
@@ -622,7 +635,7 @@ void launch_grouped_gemm(int dt, int variant, const GGDesc& g, int grid, hipStre
   OME_DISPATCH(dt, O, {
     GGDesc gg = g;
     gg.mode = epi_mode<O>(g.alpha, g.beta);
-    hipLaunchKernelGGL(k_grouped_valu<O>, dim3(grid), dim3(256), 0, st, gg);
+    hipLaunchKernelGGL(k_grouped_valu<O>, dim3(grid), dim3(256), 0, st, gg, g.probs, g.first, g.map);
   });
 }
 
