@@ -8,8 +8,10 @@
 //   2. admit: queued requests are scanned in arrival order and every one whose plan memory
 //      (arena + scratch + staging + accumulators, known exactly at plan time) fits the
 //      remaining HBM budget is started on a free slot: memory from the pool is lent to the
-//      plan, the slot's stream waits for the submitter's stream (the inputs are ready), the
-//      whole evaluation is enqueued, and a completion event is recorded;
+//      plan, the slot's stream waits for the context's stream as it stands at admission (the
+//      inputs are ready, and so is every earlier use of the lent memory: the pool hands out
+//      blocks freed on the host as soon as work on them was enqueued on the context's
+//      stream), the whole evaluation is enqueued, and a completion event is recorded;
 //   3. sleep briefly while requests run, or block until the next submission when idle.
 // Each slot has its own stream, so the steps of admitted requests overlap on the device, and
 // requests that arrive while others run are admitted as soon as a slot and memory are free.
@@ -48,6 +50,7 @@ struct Req {
 struct Slot {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
+  hipEvent_t admit = nullptr;  // recorded on the context's stream when a request is admitted
   Req* req = nullptr;
 };
 
@@ -128,8 +131,21 @@ void scheduler(ome_server* s) {
       Slot& sl = *free_slot;
       // enqueue outside the server lock (submitters keep going); the slot is claimed
       lk.unlock();
-      hipStreamWaitEvent(sl.st, r->ready, 0);
-      Status st = tree_run_lent(r->tree, r->xs.data(), r->y, mem, sl.st);
+      hipError_t ea;
+      {
+        std::lock_guard<std::recursive_mutex> g(s->ctx->mu);
+        ea = hipEventRecord(sl.admit, s->ctx->stream);
+      }
+      Status st = ea == hipSuccess
+                      ? Status::ok()
+                      : Status::err(OME_ERR_DEVICE, "could not order the request after the context's stream");
+      if (st.good()) {
+        hipStreamWaitEvent(sl.st, r->ready, 0);
+        hipStreamWaitEvent(sl.st, sl.admit, 0);
+        st = tree_run_lent(r->tree, r->xs.data(), r->y, mem, sl.st);
+      } else {
+        hipGetLastError();
+      }
       hipError_t e = hipEventRecord(sl.done, sl.st);
       lk.lock();
       if (!st.good() || e != hipSuccess) {
@@ -175,11 +191,13 @@ int ome_server_create(ome_ctx* ctx, size_t hbm_budget, int max_active, ome_serve
   s->slots.resize(max_active);
   for (auto& sl : s->slots) {
     if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.admit, hipEventDisableTiming) != hipSuccess) {
       hipGetLastError();
       for (auto& q : s->slots) {
         if (q.st) hipStreamDestroy(q.st);
         if (q.done) hipEventDestroy(q.done);
+        if (q.admit) hipEventDestroy(q.admit);
       }
       delete s;
       ctx->last_error = "could not create the request-loop streams";
@@ -270,6 +288,7 @@ int ome_server_destroy(ome_server* s) {
     hipStreamSynchronize(sl.st);
     hipStreamDestroy(sl.st);
     hipEventDestroy(sl.done);
+    hipEventDestroy(sl.admit);
   }
   for (auto& kv : s->reqs) {
     hipEventDestroy(kv.second->ready);

@@ -113,3 +113,43 @@ def test_sliced_request_through_the_loop():
         b = complex(om().to_numpy(srv.wait(t2)[0])[()])
     assert abs(a - ref) <= 1e-4 * abs(ref)
     assert abs(b - ref) <= 1e-4 * abs(ref)
+
+
+def test_admission_waits_for_reused_pool_memory():
+    # A request queued behind a busy slot is admitted later and may be lent a pool block that
+    # the caller freed right after enqueuing writes to it on the context's stream (the pool
+    # reuses blocks in the order of that stream).  Admission orders the request after the
+    # context's stream, so those writes land before the request uses the block: its result
+    # equals the same request run alone.
+    import ctypes
+    from omeinsum_mi355x import _lib
+    from omeinsum_mi355x.server import Server, plan_bytes
+    tree0, sd0, xs0, dxs0 = request(60, 5401)
+    tree, sd, xs, dxs = request(44, 5402)
+    g = torch.Generator(device="cuda").manual_seed(5403)
+    big = torch.randn(4096, 4096, device="cuda", generator=g)
+    with Server(hbm_budget=1 << 30, max_active=1) as srv:
+        p0 = srv.plan(tree0, dxs0, sd0)
+        p = srv.plan(tree, dxs, sd)
+        alone = complex(om().to_numpy(srv.wait(srv.submit(p, dxs))[0])[()])
+        ctx = srv.ctx
+        nb = plan_bytes(p)
+        garbage = torch.full((nb // 4 + 1,), float("nan"), device="cuda")
+        blk = ctypes.c_void_p()
+        _lib.check(ctx.lib.ome_alloc(ctx.h, nb, ctypes.byref(blk)), ctx.h)
+        t0 = srv.submit(p0, dxs0)
+        t1 = srv.submit(p, dxs)  # queued behind t0: one slot
+        acc = torch.zeros_like(big)
+        for _ in range(20):  # milliseconds of work on the context's stream, then the writes
+            acc += big @ big
+            _lib.check(ctx.lib.ome_memcpy_d2d(ctx.h, blk, ctypes.c_void_p(garbage.data_ptr()),
+                                              nb), ctx.h)
+        _lib.check(ctx.lib.ome_free(ctx.h, blk), ctx.h)
+        y0, _ = srv.wait(t0)
+        y1, _ = srv.wait(t1)
+        torch.cuda.synchronize()
+        got = complex(om().to_numpy(y1)[()])
+        assert abs(got - alone) <= 1e-6 * abs(alone), (got, alone)
+        ref0 = fp64(tree0, sd0, xs0)
+        got0 = complex(om().to_numpy(y0)[()])
+        assert abs(got0 - ref0) <= 1e-4 * abs(ref0), (got0, ref0)
