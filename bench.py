@@ -23,8 +23,9 @@ program omeinsum.jl_amd/calib/mfma_loop on random operands; the matrix cores dra
 on random data and the part lowers its clock, so this is the rate the loop can reach on real
 data, and frac there is achieved / that rate.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|c4|c1|c2|c3|c5]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|head|c4|c1|c2|c3|c5]
                        [--no-cpu] [--copy-calibration]
+(--workload head runs only the headline BF16x3 product, to profile its kernel alone.)
 """
 import argparse
 import json
@@ -627,8 +628,12 @@ def main():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", choices=("all", "c4", "c1", "c2", "c3", "c5"), default="all")
+    # the part's power management holds a lower clock for the first tens of back-to-back
+    # launches after an idle period (headline product: 0.40 ms/step after 5 warm-up steps,
+    # 0.35 ms/step after 50, with the same kernels), so the default warms up past that
+    p.add_argument("--warmup", type=int, default=50)
+    p.add_argument("--workload", choices=("all", "head", "c4", "c1", "c2", "c3", "c5"),
+                   default="all")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--copy-calibration", action="store_true")
     args = p.parse_args()
