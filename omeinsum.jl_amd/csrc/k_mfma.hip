@@ -239,20 +239,33 @@ __global__ __launch_bounds__(256) void k_mfma_bf16x3(GemmDesc d, int64_t tiles_m
 }
 
 // ---- FP32 ------------------------------------------------------------------------------------
-constexpr int F32_BK = 16;
+constexpr int F32_BK = 32;
 constexpr int F32_LDK = F32_BK + 4;
 
+// Stage one thread's F32_BK/2 values (the layout of load_f32_tile) into a 128 x F32_BK tile.
 template <bool KMAJ>
 __device__ __forceinline__ void store_f32(float (*__restrict__ s)[F32_LDK], const float* __restrict__ v) {
   const int t = threadIdx.x;
   if constexpr (KMAJ) {
-    const int r = t >> 1, kb = (t & 1) * 8;
-    *(float4*)&s[r][kb] = make_float4(v[0], v[1], v[2], v[3]);
-    *(float4*)&s[r][kb + 4] = make_float4(v[4], v[5], v[6], v[7]);
-  } else {
-    const int r4 = (t & 31) * 4, kb = (t >> 5) * 2;
+    constexpr int NV = F32_BK / 2;
+    const int r = t >> 1, kb = (t & 1) * NV;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) *(float2*)&s[r4 + i][kb] = make_float2(v[i], v[4 + i]);
+    for (int i = 0; i < NV; i += 4)
+      *(float4*)&s[r][kb + i] = make_float4(v[i], v[i + 1], v[i + 2], v[i + 3]);
+  } else {
+    constexpr int NK = F32_BK / 8;
+    const int r4 = (t & 31) * 4, kb = (t >> 5) * NK;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if constexpr (NK == 2) {
+        *(float2*)&s[r4 + i][kb] = make_float2(v[i], v[4 + i]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < NK; j += 4)
+          *(float4*)&s[r4 + i][kb + j] =
+              make_float4(v[4 * j + i], v[4 * (j + 1) + i], v[4 * (j + 2) + i], v[4 * (j + 3) + i]);
+      }
+    }
   }
 }
 
@@ -276,7 +289,7 @@ __global__ __launch_bounds__(256) void k_mfma_f32(GemmDesc d, int64_t tiles_m, i
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
-  float ra[8], rb[8];
+  float ra[F32_BK / 2], rb[F32_BK / 2];
   load_f32_tile<AK, F32_BK>(A, m0, kbeg, d.M, kend, d.sam, d.sak, vec_a, ra);
   load_f32_tile<BK, F32_BK>(B, n0, kbeg, d.N, kend, d.sbn, d.sbk, vec_b, rb);
   for (int64_t k0 = kbeg; k0 < kend; k0 += F32_BK) {
@@ -321,6 +334,7 @@ __global__ __launch_bounds__(256) void k_mfma_f32(GemmDesc d, int64_t tiles_m, i
 // so C' = A' B' holds Cr in the re rows and Ci in the im rows, and the two 32-wide blocks of a
 // wave are the real and imaginary parts of the same 32 x 32 outputs.
 constexpr int C32_BK = 8;
+constexpr int C32_LDK = 2 * C32_BK + 4;  // real k per LDS row: [re | im] + 4 pad
 
 // A tile: 128 rows x 8 k, 4 complex per thread.
 //   KMAJ (sk == 1): row t / 2, k = (t % 2) * 4 + i;  !KMAJ (sr == 1): row t % 128, k = (t / 128) * 4 + i
@@ -359,8 +373,8 @@ __device__ __forceinline__ void load_c32_b(const c32* __restrict__ P, int64_t r0
 template <bool AK, bool BK>
 __global__ __launch_bounds__(256) void k_mfma_c32(GemmDesc d, int64_t tiles_m, int64_t tiles_n,
                                                   int64_t kchunk) {
-  __shared__ __attribute__((aligned(16))) float sA[128][F32_LDK];
-  __shared__ __attribute__((aligned(16))) float sB[128][F32_LDK];
+  __shared__ __attribute__((aligned(16))) float sA[128][C32_LDK];
+  __shared__ __attribute__((aligned(16))) float sB[128][C32_LDK];
   const TileIdx ti = map_tile(blockIdx.x, gridDim.x, tiles_m, tiles_n, d.B);
   const int64_t m0 = ti.tm * 128, n0 = ti.tn * 64;
   const int64_t kbeg = ti.ks * kchunk;
@@ -633,7 +647,7 @@ void launch_gemm_mfma(int prec, GemmDesc d, int ksplit, hipStream_t st) {
   const bool f64 = (prec == PREC_FP64);
   d.mode = f64 ? epi_mode<OpsF64>(d.sx, d.sy) : epi_mode<OpsF32>(d.sx, d.sy);
   if (ksplit < 1) ksplit = 1;
-  const int kstep = (prec == PREC_BF16X3) ? 32 : 16;
+  const int kstep = (prec == PREC_BF16X3) ? B3_BK : f64 ? F64_BK : F32_BK;
   int64_t kchunk = (d.K + ksplit - 1) / ksplit;
   kchunk = ((kchunk + kstep - 1) / kstep) * kstep;
   if (kchunk < kstep) kchunk = kstep;
