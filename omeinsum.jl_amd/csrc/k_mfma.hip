@@ -467,35 +467,38 @@ __global__ __launch_bounds__(256) void k_mfma_c32(GemmDesc d, int64_t tiles_m, i
 }
 
 // ---- FP64 ------------------------------------------------------------------------------------
-constexpr int F64_BK = 16;
+constexpr int F64_BK = 32;
 constexpr int F64_LDK = F64_BK + 2;
 
 template <bool KMAJ>
 __device__ __forceinline__ void load_f64_tile(const double* __restrict__ P, int64_t r0, int64_t k0,
                                               int64_t R, int64_t K, int64_t sr, int64_t sk,
                                               bool vec, double* __restrict__ v) {
-  // 128 rows x 16 k = 2048 values, 8 per thread.
-  // KMAJ: row = t/2, k = (t&1)*8 + i.  !KMAJ: rows r2, r2+1 with r2 = (t&63)*2, k = (t>>6)*4 + j
+  // 128 rows x F64_BK k, F64_BK / 2 values per thread.
+  // KMAJ: row = t/2, k = (t&1)*(F64_BK/2) + i.
+  // !KMAJ: rows r2, r2+1 with r2 = (t&63)*2, k = (t>>6)*(F64_BK/4) + j.
   const int t = threadIdx.x;
   if constexpr (KMAJ) {
+    constexpr int NV = F64_BK / 2;
     const int64_t r = r0 + (t >> 1);
-    const int64_t kb = k0 + (t & 1) * 8;
+    const int64_t kb = k0 + (t & 1) * NV;
     const double* p = P + r * sr + kb;
-    if (vec && r < R && kb + 8 <= K) {
+    if (vec && r < R && kb + NV <= K) {
 #pragma unroll
-      for (int i = 0; i < 8; i += 2) {
+      for (int i = 0; i < NV; i += 2) {
         double2 q = *(const double2*)(p + i);
         v[i] = q.x; v[i + 1] = q.y;
       }
     } else {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = (r < R && kb + i < K) ? p[i] : 0.0;
+      for (int i = 0; i < NV; ++i) v[i] = (r < R && kb + i < K) ? p[i] : 0.0;
     }
   } else {
+    constexpr int NK = F64_BK / 4;
     const int64_t r2 = r0 + (t & 63) * 2;
-    const int64_t kb = k0 + (t >> 6) * 4;
+    const int64_t kb = k0 + (t >> 6) * NK;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NK; ++j) {
       const double* p = P + r2 + (kb + j) * sk;
       if (vec && r2 + 2 <= R && kb + j < K) {
         double2 q = *(const double2*)p;
@@ -512,16 +515,18 @@ template <bool KMAJ>
 __device__ __forceinline__ void store_f64(double (*__restrict__ s)[F64_LDK], const double* __restrict__ v) {
   const int t = threadIdx.x;
   if constexpr (KMAJ) {
-    const int r = t >> 1, kb = (t & 1) * 8;
+    constexpr int NV = F64_BK / 2;
+    const int r = t >> 1, kb = (t & 1) * NV;
 #pragma unroll
-    for (int i = 0; i < 8; i += 2) *(double2*)&s[r][kb + i] = make_double2(v[i], v[i + 1]);
+    for (int i = 0; i < NV; i += 2) *(double2*)&s[r][kb + i] = make_double2(v[i], v[i + 1]);
   } else {
-    const int r2 = (t & 63) * 2, kb = (t >> 6) * 4;
+    constexpr int NK = F64_BK / 4;
+    const int r2 = (t & 63) * 2, kb = (t >> 6) * NK;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      *(double2*)&s[r2 + i][kb] = make_double2(v[i], v[2 + i]);
-      *(double2*)&s[r2 + i][kb + 2] = make_double2(v[4 + i], v[6 + i]);
-    }
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < NK; j += 2)
+        *(double2*)&s[r2 + i][kb + j] = make_double2(v[2 * j + i], v[2 * (j + 1) + i]);
   }
 }
 
@@ -545,7 +550,7 @@ __global__ __launch_bounds__(256) void k_mfma_f64(GemmDesc d, int64_t tiles_m, i
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0;
-  double ra[8], rb[8];
+  double ra[F64_BK / 2], rb[F64_BK / 2];
   load_f64_tile<AK>(A, m0, kbeg, d.M, kend, d.sam, d.sak, vec_a, ra);
   load_f64_tile<BK>(B, n0, kbeg, d.N, kend, d.sbn, d.sbk, vec_b, rb);
   for (int64_t k0 = kbeg; k0 < kend; k0 += F64_BK) {
