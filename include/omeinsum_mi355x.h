@@ -285,7 +285,9 @@ int ome_tree_destroy(ome_tree* tree);
  * Continuous batching of tensor-network contraction requests: a request is admitted when
  * its tree's arena fits the remaining HBM budget and an execution slot is free; admitted
  * requests run concurrently on separate streams; requests that complete free their slot
- * and arena so waiting requests are admitted as they arrive. */
+ * and arena so waiting requests are admitted as they arrive.  Admission orders a request
+ * after the work on the context's stream at that moment (an event recorded there), so that
+ * stream must not be under graph capture while requests are queued. */
 int ome_server_create(ome_ctx* ctx, size_t hbm_budget, int max_active, ome_server** out);
 /* xs and y stay owned by the caller and must stay valid until the request completes. */
 int ome_server_submit(ome_server* srv, ome_tree* tree, const void* const* xs, void* y,

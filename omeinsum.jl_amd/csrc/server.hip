@@ -16,7 +16,8 @@
 // Each slot has its own stream, so the steps of admitted requests overlap on the device, and
 // requests that arrive while others run are admitted as soon as a slot and memory are free.
 // Sliced requests (SlicedEinsum plans) keep their FP64 cross-slice accumulator in the lent
-// memory.  A tree must not be used elsewhere while its request is queued or running.
+// memory.  A tree must not be used elsewhere while its request is queued or running, and the
+// context's stream must not be under graph capture while the loop has queued requests.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
