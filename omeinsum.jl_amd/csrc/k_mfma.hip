@@ -530,8 +530,11 @@ __device__ __forceinline__ void store_f64(double (*__restrict__ s)[F64_LDK], con
   }
 }
 
+// Two waves per SIMD: the 128 FP64 accumulator registers per lane plus operands fit 256
+// registers, and a second resident workgroup covers the other's barriers and loads (left to
+// itself the compiler takes 338 registers and one wave per SIMD, 18% slower on 4096^3).
 template <bool AK, bool BK>
-__global__ __launch_bounds__(256) void k_mfma_f64(GemmDesc d, int64_t tiles_m, int64_t tiles_n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma_f64(GemmDesc d, int64_t tiles_m, int64_t tiles_n,
                                                   int64_t kchunk, int vec_a, int vec_b) {
   __shared__ __attribute__((aligned(16))) double sA[128][F64_LDK];
   __shared__ __attribute__((aligned(16))) double sB[128][F64_LDK];
