@@ -2,8 +2,8 @@
  * omeinsum_mi355x.h — C ABI of the MI355X (gfx950) contraction backend for OMEinsum.
  *
  * Every entry point takes plain pointers and sizes; no Julia, Python or torch types cross
- * this boundary.  The Julia glue binds these with `ccall` (INTEGRATION.md shows the stubs);
- * the Python host package binds them with ctypes.
+ * this boundary.  A Julia binding calls them with `ccall`; the Python host package binds
+ * them with ctypes.
  *
  * Conventions shared by all entry points
  *   - Tensors are dense and COLUMN-MAJOR (first label fastest), as Julia arrays are.
