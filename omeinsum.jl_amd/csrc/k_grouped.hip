@@ -229,8 +229,12 @@ __device__ __forceinline__ void gg_split(float x, uint16_t& hi, uint16_t& lo) {
   lo = gg_f2bf(x - __uint_as_float((uint32_t)hi << 16));
 }
 
+// 64 x 64 tiles at five waves per SIMD (at most 102 VGPRs; 90 without spills): the many short
+// tiles are latency-bound, and the fifth resident workgroup took the 4096-problem launch from
+// 3.15 to 2.84 ms against the compiler's own choice of 106 VGPRs.  128 x 128 tiles keep the
+// compiler's choice.
 template <int TM, int TN>
-__global__ __launch_bounds__(256) void k_grouped_bf16x3(GGDesc g, const GGProb* __restrict__ probs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 ? 5 : 1, TM == 64 ? 5 : 8))) void k_grouped_bf16x3(GGDesc g, const GGProb* __restrict__ probs,
                                                         const int64_t* __restrict__ first,
                                                         const int32_t* __restrict__ map) {
   constexpr int BK = 32, LDK = 40;
