@@ -270,17 +270,25 @@ def bench_c2(steps, warmup):
     dxs = [torch.from_numpy(x).cuda() for x in xs]
     # FP64 reference of the same tree (the oracle's per-step restatement, in complex128)
     ref = complex(db.nested_einsum(tree.to_plain(), [x.astype(np.complex128) for x in xs], sd)[()])
+    tflop = 8.0 * co.flop(tree, sd) / 1e12
     out = {"n_vertices": n, "flops_log2": float(np.log2(8 * co.flop(tree, sd)))}
     y = torch.empty((), dtype=torch.complex64, device="cuda")
-    for name, be in (("mi355x_fp32", om.MI355XBackend(precision="fp32")),
-                     ("mi355x_bf16x3", om.MI355XBackend(precision="bf16x3")),
-                     ("per_step_fp32", om.DefaultBackend())):
+    dxs128 = [torch.from_numpy(x.astype(np.complex128)).cuda() for x in xs]
+    y128 = torch.empty((), dtype=torch.complex128, device="cuda")
+    for name, be, ins, out_y in (
+            ("mi355x_fp32", om.MI355XBackend(precision="fp32"), dxs, y),
+            ("mi355x_bf16x3", om.MI355XBackend(precision="bf16x3"), dxs, y),
+            ("mi355x_fp64_precision", om.MI355XBackend(precision="fp64"), dxs, y),
+            ("mi355x_complexf64", om.MI355XBackend(), dxs128, y128),
+            ("per_step_fp32", om.DefaultBackend(), dxs, y)):
         om.set_einsum_backend(be)
-        fn = lambda: om.einsum_inplace(tree, dxs, y, True, False, sd)
+        fn = lambda: om.einsum_inplace(tree, ins, out_y, True, False, sd)
         ms = time_events(fn, steps, warmup)
         fn()
-        got = complex(y.cpu().numpy()[()])
-        out[name] = {"ms": ms, "rel_err_fp64": abs(got - ref) / abs(ref)}
+        got = complex(out_y.cpu().numpy()[()])
+        out[name] = {"ms": ms, "tflops": tflop / ms * 1e3,
+                     "frac_fp32_mfma_peak": tflop / ms * 1e3 / PEAK_FP32,
+                     "rel_err_fp64": abs(got - ref) / abs(ref)}
     om.set_einsum_backend(om.MI355XBackend())
     plan = plan_for(tree, dxs, sd)
     info = plan.info()
@@ -288,10 +296,18 @@ def bench_c2(steps, warmup):
                    "arena_bytes": int(info.arena_bytes), "flops": info.flops}
     v, lg = plan.execute_scalar_normalized(dxs)
     out["normalized_rel_err_fp64"] = abs(v * np.exp(lg) - ref) / abs(ref)
-    # the CPU reference path: the oracle's per-step restatement in ComplexF32
-    t0 = time.perf_counter()
+    # the CPU reference path: the oracle's per-step restatement in ComplexF32, median of up
+    # to 20 runs after one warm-up (at most ~10 s)
     db.nested_einsum(tree.to_plain(), xs, sd)
-    out["cpu_ms"] = (time.perf_counter() - t0) * 1e3
+    ts, t_end = [], time.time() + 10
+    for _ in range(20):
+        t0 = time.perf_counter()
+        db.nested_einsum(tree.to_plain(), xs, sd)
+        ts.append(time.perf_counter() - t0)
+        if time.time() > t_end:
+            break
+    out["cpu_ms"] = statistics.median(ts) * 1e3
+    out["cpu_runs"] = len(ts)
     om.set_einsum_backend(om.DefaultBackend())
     log("C2 100-vertex network: %s" % {k: v for k, v in out.items() if isinstance(v, dict)})
     return out
@@ -301,34 +317,16 @@ def bench_c3(steps, warmup):
     """4096 uneven batched GEMMs (ein"ijb,jkb->ikb", i, j, k in {8, 16, ..., 256}, b in 1..16,
     Float32) in one grouped launch, plus 1024 ein"ijkl->ljik" permutes (dims in [4, 64]) in
     one grouped launch; against the per-problem path (one library call per GEMM / permute)."""
-    import numpy as np
     import torch
     import omeinsum_mi355x as om
     from omeinsum_mi355x import _lib
+    from omeinsum_mi355x.workloads import uneven_batch
 
-    r = np.random.Generator(np.random.PCG64(3000))
-    g = torch.Generator(device="cuda").manual_seed(3001)
-    dims8 = np.arange(8, 257, 8)
-    probs, flops, gemm_bytes = [], 0.0, 0.0
-    for _ in range(4096):
-        i, j, k = (int(x) for x in r.choice(dims8, 3))
-        b = int(r.integers(1, 17))
-        # column-major A[i,j,b], B[j,k,b], C[i,k,b] are row-major (b,j,i), (b,k,j), (b,k,i):
-        # C_t = B_t @ A_t in torch's batch-major convention
-        At = torch.rand(b, j, i, device="cuda", generator=g) * 2 - 1
-        Bt = torch.rand(b, k, j, device="cuda", generator=g) * 2 - 1
-        Ct = torch.empty(b, k, i, device="cuda")
-        probs.append((Bt, At, Ct))
-        flops += 2.0 * i * j * k * b
-        gemm_bytes += 4.0 * b * (i * j + j * k + i * k)
-    xs, ys, perm_bytes = [], [], 0.0
-    for _ in range(1024):
-        i, j, k, l = (int(x) for x in r.integers(4, 65, 4))
-        # column-major x[i,j,k,l] -> y[l,j,i,k] is row-major (l,k,j,i) -> (k,i,j,l)
-        x = torch.rand(l, k, j, i, device="cuda", generator=g)
-        xs.append(x)
-        ys.append(torch.empty(k, i, j, l, device="cuda"))
-        perm_bytes += 8.0 * i * j * k * l
+    # every operand is a view of one of two device buffers (a few dispatches build them)
+    probs, pairs, info = uneven_batch(seed=3000)
+    flops, gemm_bytes, perm_bytes = info["gemm_flop"], info["gemm_bytes"], info["permute_bytes"]
+    xs = [x for x, _ in pairs]
+    ys = [y for _, y in pairs]
     perms = [(1, 3, 2, 0)] * len(xs)
     gg = om.GroupedGemm(probs)
     gp = om.GroupedPermute(xs, perms, ys)
@@ -405,44 +403,46 @@ def bench_c3(steps, warmup):
     out["cpu_sample"] = {"gemm_tflops": fs / t_g / 1e12, "permute_gbs": ps / t_p / 1e9,
                          "whole_step_ms_est": (flops / (fs / t_g) + perm_bytes / (ps / t_p)) * 1e3,
                          "sample": "every 16th instance and permute, numpy/OpenBLAS"}
-    del probs, xs, ys, gg, gp
+    del probs, pairs, info, xs, ys, gg, gp
     torch.cuda.empty_cache()
     return out
 
 
 def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
     """Serving stream: requests are random 3-regular graphs with an even number of vertices in
-    [40, 140], bond dimension 2, ComplexF32 Gaussian tensors, greedy order; a request whose
-    largest intermediate exceeds 2^sc_budget elements (2^25 ComplexF32 = 256 MiB, a plan of
-    about 1 GiB) is sliced to fit (FP64 cross-slice sum).
+    [40, 140], bond dimension 2, ComplexF32 Gaussian tensors, greedy order (the request list of
+    networks.request_stream, seed 5000); a request whose largest intermediate exceeds
+    2^sc_budget elements (2^25 ComplexF32 = 256 MiB, a plan of about 1 GiB) is sliced to fit
+    (FP64 cross-slice sum).
     Inputs and plans are prepared before the timed runs (planning time reported apart).
-    Measured: the plain per-step FP32 path back to back, the request loop (BF16x3 split
-    products on the matrix cores) at saturation (every request submitted at once), and the
-    loop under Poisson arrivals at 70% of its saturation rate (p50 / p99 latency from
-    submission to completion); the loop's error against FP64 on the requests the CPU oracle
-    contracts quickly."""
+    Measured: the plain per-step FP32 path back to back, the request loop at saturation (every
+    request submitted at once) in FP32 and in BF16x3, and each loop under Poisson arrivals at
+    70% of its saturation rate (p50 / p99 latency from submission to completion).  Every request's result
+    is compared with its FP64 value (tests/golden/request_stream_48.json, the oracle's
+    contraction of the same network in complex128)."""
     import numpy as np
     import torch
     import omeinsum_mi355x as om
     from omeinsum_mi355x import contractionorder as co, networks
     from omeinsum_mi355x.server import Server
 
-    r = np.random.Generator(np.random.PCG64(5000))
-    reqs = []
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "request_stream_48.json")))
+    reqs, refs = [], []
     t_plan = time.perf_counter()
     flops = 0.0
-    for _ in range(nreq):
-        n = 2 * int(r.integers(20, 71))
-        edges = networks.random_regular_graph(n, 3, seed=int(r.integers(1 << 30)))
-        code = networks.network_code(n, edges)
-        sd = {e: 2 for e in range(len(edges))}
-        xs = networks.gaussian_tensors(code, sd, int(r.integers(1 << 30)), np.complex64)
+    for spec, fq in zip(networks.request_stream(nreq, fx["seed"]), fx["requests"]):
+        code = networks.network_code(spec["n"], spec["edges"])
+        sd = {e: 2 for e in range(len(spec["edges"]))}
+        xs = networks.gaussian_tensors(code, sd, spec["tensor_seed"], np.complex64)
         tree = co.optimize_code(code, sd)
+        if co.flop(tree, sd) != fq["flop"]:
+            raise RuntimeError("request tree differs from the fixture's")
         if co.contraction_complexity(tree, sd)[1] > sc_budget:
             tree = co.slice_code(tree, sd, sc_target=sc_budget)
         f = 8.0 * co.flop(tree, sd)
         flops += f
-        reqs.append((n, tree, sd, [torch.from_numpy(x).cuda() for x in xs], f))
+        reqs.append((spec["n"], tree, sd, [torch.from_numpy(x).cuda() for x in xs], f))
+        refs.append(complex(*fq["value_fp64"]))
     t_plan = time.perf_counter() - t_plan
     sliced = sum(1 for q in reqs if hasattr(q[1], "slicing") and q[1].slicing)
     out = {"requests": nreq, "sliced": sliced, "sc_budget_log2_elems": sc_budget,
@@ -450,6 +450,11 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
 
     def scalar(y):
         return complex(om.to_numpy(y)[()])
+
+    def err_stats(vals):
+        e = [abs(v - r) / abs(r) for v, r in zip(vals, refs)]
+        return {"checked": len(e), "max": max(e), "median": float(np.median(e)),
+                "worst_request": int(np.argmax(e))}
 
     # 1. the plain per-step path (one einsum per tree node from the host, new output per node)
     om.set_einsum_backend(om.DefaultBackend())
@@ -464,14 +469,15 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
         lat.append((time.perf_counter() - a) * 1e3)
     t_ps = time.perf_counter() - t0
     out["per_step"] = {"req_per_s": nreq / t_ps, "p50_ms": float(np.percentile(lat, 50)),
-                       "p99_ms": float(np.percentile(lat, 99)), "tflops": flops / t_ps / 1e12}
-    log("C5 per-step path: %.1f req/s (p50 %.2f ms, p99 %.2f ms)" %
-        (nreq / t_ps, out["per_step"]["p50_ms"], out["per_step"]["p99_ms"]))
+                       "p99_ms": float(np.percentile(lat, 99)), "tflops": flops / t_ps / 1e12,
+                       "rel_err_fp64": err_stats(per_step)}
+    log("C5 per-step path: %.1f req/s (p50 %.2f ms, p99 %.2f ms); error vs FP64 median %.1e max %.1e"
+        % (nreq / t_ps, out["per_step"]["p50_ms"], out["per_step"]["p99_ms"],
+           out["per_step"]["rel_err_fp64"]["median"], out["per_step"]["rel_err_fp64"]["max"]))
 
-    def run_loop(prec):
+    def run_loop(backend):
         res_out = {}
-        with Server(hbm_budget=hbm_budget, max_active=max_active,
-                    backend=om.MI355XBackend(precision=prec)) as srv:
+        with Server(hbm_budget=hbm_budget, max_active=max_active, backend=backend) as srv:
             t0 = time.perf_counter()
             plans = [srv.plan(tree, dxs, sd) for n, tree, sd, dxs, f in reqs]
             res_out["device_planning_s"] = time.perf_counter() - t0
@@ -507,43 +513,34 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
                                   "p50_ms": float(np.percentile(lats, 50)),
                                   "p99_ms": float(np.percentile(lats, 99))}
         res_out["speedup_req_per_s"] = sat_rps / out["per_step"]["req_per_s"]
-        return res_out, vals
+        res_out["rel_err_fp64"] = err_stats(vals)
+        res_out["max_rel_diff_vs_per_step"] = max(abs(a - b) / abs(b) for a, b in zip(vals, per_step))
+        return res_out
 
-    # error against FP64 (the oracle's per-step restatement in complex128) for every request
-    # whose tree the CPU contracts in well under a second
+    loops = (("fp32", om.MI355XBackend(precision="fp32")),
+             ("bf16x3", om.MI355XBackend(precision="bf16x3")))
+    for name, be in loops:
+        res_out = run_loop(be)
+        out["loop_" + name] = res_out
+        log("C5 request loop (%s): %.1f req/s saturated (%.2fx the per-step path); Poisson at %.1f "
+            "req/s: p50 %.2f ms, p99 %.2f ms; error vs FP64 over %d requests: median %.1e max %.1e" %
+            (name, res_out["saturated"]["req_per_s"], res_out["speedup_req_per_s"],
+             res_out["poisson"]["offered_req_per_s"], res_out["poisson"]["p50_ms"],
+             res_out["poisson"]["p99_ms"], res_out["rel_err_fp64"]["checked"],
+             res_out["rel_err_fp64"]["median"], res_out["rel_err_fp64"]["max"]))
+    # the CPU reference path on the small requests (oracle per-step restatement, FP64), timed
     from oracle import default_backend as db
-    refs = {}
-    t_cpu = 0.0
+    t_cpu, n_cpu = 0.0, 0
     for q, (n, tree, sd, dxs, f) in enumerate(reqs):
         if f <= 2e9:
             plain = tree.eins if hasattr(tree, "slicing") else tree
             xs64 = [x.cpu().numpy().astype(np.complex128) for x in dxs]
             t1 = time.perf_counter()
-            refs[q] = complex(db.nested_einsum(plain.to_plain(), xs64, sd)[()])
+            db.nested_einsum(plain.to_plain(), xs64, sd)
             t_cpu += time.perf_counter() - t1
-    # the CPU reference path on the same small requests (oracle per-step restatement, FP64)
-    out["cpu_small_requests"] = {"requests": len(refs), "req_per_s": len(refs) / t_cpu,
-                                 "sample": "requests of <= 2e9 flop, ComplexF64, one host thread "
-                                           "per GEMM call (numpy/OpenBLAS)"}
-
-    def err_stats(vals):
-        e = [abs(vals[q] - r) / abs(r) for q, r in refs.items()]
-        return {"checked": len(e), "max": max(e), "median": float(np.median(e))}
-
-    out["per_step"]["rel_err_fp64"] = err_stats(per_step)
-    for prec in ("fp32", "bf16x3"):
-        res_out, vals = run_loop(prec)
-        res_out["rel_err_fp64"] = err_stats(vals)
-        res_out["max_rel_diff_vs_per_step"] = max(abs(a - b) / abs(b) for a, b in zip(vals, per_step))
-        out["loop_" + prec] = res_out
-        log("C5 request loop (%s): %.1f req/s saturated (%.2fx the per-step path); Poisson at %.1f "
-            "req/s: p50 %.2f ms, p99 %.2f ms; error vs FP64 median %.1e max %.1e" %
-            (prec, res_out["saturated"]["req_per_s"], res_out["speedup_req_per_s"],
-             res_out["poisson"]["offered_req_per_s"], res_out["poisson"]["p50_ms"],
-             res_out["poisson"]["p99_ms"], res_out["rel_err_fp64"]["median"],
-             res_out["rel_err_fp64"]["max"]))
-    log("C5 per-step path error vs FP64: median %.1e max %.1e" %
-        (out["per_step"]["rel_err_fp64"]["median"], out["per_step"]["rel_err_fp64"]["max"]))
+            n_cpu += 1
+    out["cpu_small_requests"] = {"requests": n_cpu, "req_per_s": n_cpu / t_cpu,
+                                 "sample": "requests of <= 2e9 flop, ComplexF64, numpy/OpenBLAS"}
     return out
 
 
@@ -575,7 +572,7 @@ def blas_threads():
 def cpu_baseline(ab):
     """The oracle's restatement of OMEinsum's DefaultBackend (analyze_binary + permutes +
     OpenBLAS GEMM through numpy) on this host, on the same 64^4 Float32 inputs.  Bounded
-    sample: up to 5 timed repetitions after one warm-up, stopping after ~25 s."""
+    sample: up to 20 timed repetitions after one warm-up, stopping after ~25 s."""
     from oracle import default_backend as db
     a = ab[0].cpu().numpy()
     b = ab[1].cpu().numpy()
@@ -583,7 +580,7 @@ def cpu_baseline(ab):
     db.einsum(ixs, iy, [a, b])
     ts = []
     t_end = time.time() + 25
-    for _ in range(5):
+    for _ in range(20):
         t0 = time.perf_counter()
         db.einsum(ixs, iy, [a, b])
         ts.append(time.perf_counter() - t0)
@@ -592,9 +589,21 @@ def cpu_baseline(ab):
     med = statistics.median(ts)
     return {"value": round(C4_FLOPS / med / 1e12, 4), "unit": "TFLOP/s", "cores": blas_threads(),
             "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": "the full abcd,cdef->abef 64^4 Float32 contraction, oracle DefaultBackend "
                       "restatement (numpy permutes + OpenBLAS sgemm), median of %d runs = %.3f s"
                       % (len(ts), med)}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def mfma_calibration(steps=128):
@@ -694,20 +703,27 @@ def main():
     }
     c5 = extra.get("c5_serving")
     if c5:
-        # the tensor-network request stream: request loop (BF16x3 split products, and FP32)
-        # vs the plain per-step FP32 GPU path
-        def summary(r):
-            return {"req_per_s": round(r["saturated"]["req_per_s"], 1),
+        # the tensor-network request stream: the fastest request loop whose error against FP64
+        # is within 1e-5 on every one of the 48 requests, vs the plain per-step FP32 GPU path
+        def summary(name, r):
+            return {"loop": name, "req_per_s": round(r["saturated"]["req_per_s"], 1),
                     "vs_per_step": round(r["speedup_req_per_s"], 2),
                     "p50_ms": round(r["poisson"]["p50_ms"], 3),
                     "p99_ms": round(r["poisson"]["p99_ms"], 3),
                     "offered_req_per_s": round(r["poisson"]["offered_req_per_s"], 1),
+                    "tflops": round(r["saturated"]["tflops"], 2),
+                    "frac_fp32_mfma_peak": round(r["saturated"]["tflops"] / PEAK_FP32, 4),
+                    "rel_err_fp64_checked": r["rel_err_fp64"]["checked"],
                     "rel_err_fp64_median": r["rel_err_fp64"]["median"],
                     "rel_err_fp64_max": r["rel_err_fp64"]["max"]}
-        line["tn_requests"] = dict(summary(c5["loop_bf16x3"]),
+        loops = {k[5:]: v for k, v in c5.items() if k.startswith("loop_")}
+        ok = [k for k, v in loops.items() if v["rel_err_fp64"]["max"] <= 1e-5]
+        best = max(ok or loops, key=lambda k: loops[k]["saturated"]["req_per_s"])
+        line["tn_requests"] = dict(summary(best, loops[best]),
+                                   meets_1e5=best in ok,
                                    per_step_req_per_s=round(c5["per_step"]["req_per_s"], 1),
                                    per_step_rel_err_fp64_max=c5["per_step"]["rel_err_fp64"]["max"],
-                                   fp32_loop=summary(c5["loop_fp32"]))
+                                   other_loops=[summary(k, v) for k, v in loops.items() if k != best])
     if calib and kt:
         # the clock the part sustains under this kernel's MFMA load with real (random) data
         sus = calib["random_bf16x3_tflops"]

@@ -93,6 +93,28 @@ struct OpsC64 {
   __host__ __device__ static T from(const Scal& s) { return {s.re, s.im}; }
   static bool is_zero(const Scal& s) { return s.re == 0.0 && s.im == 0.0; }
 };
+// Accumulation arithmetic of the long reductions that are bound by HBM (the few-outputs dot
+// kernel and the split-K partial sums): Float32 and ComplexF32 sum in FP64, which costs
+// nothing there, and round once; every other type sums in its own arithmetic.
+template <class O>
+struct AccOps {
+  using A = O;
+  __host__ __device__ static typename O::T widen(typename O::T x) { return x; }
+  __host__ __device__ static typename O::T narrow(typename O::T x) { return x; }
+};
+template <>
+struct AccOps<OpsF32> {
+  using A = OpsF64;
+  __host__ __device__ static double widen(float x) { return (double)x; }
+  __host__ __device__ static float narrow(double x) { return (float)x; }
+};
+template <>
+struct AccOps<OpsC32> {
+  using A = OpsC64;
+  __host__ __device__ static c64 widen(c32 x) { return {(double)x.x, (double)x.y}; }
+  __host__ __device__ static c32 narrow(c64 x) { return {(float)x.x, (float)x.y}; }
+};
+
 // Int64: two's-complement wrapping arithmetic (Julia Int64), computed on uint64 so that
 // overflow is defined.
 struct OpsI64 {

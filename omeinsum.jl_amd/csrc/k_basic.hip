@@ -355,7 +355,7 @@ void launch_permute(int dt, const PermDesc& d, hipStream_t st) {
 // an odometer that updates every input offset incrementally.
 
 template <class O>
-__device__ __forceinline__ typename O::T walk_point(const WalkDesc& d, int64_t yi, int64_t ibeg,
+__device__ __forceinline__ typename AccOps<O>::A::T walk_point(const WalkDesc& d, int64_t yi, int64_t ibeg,
                                                     int64_t iend, bool& on_diag) {
   using T = typename O::T;
   int64_t off[WALK_MAXIN];
@@ -376,7 +376,9 @@ __device__ __forceinline__ typename O::T walk_point(const WalkDesc& d, int64_t y
       on_diag = false;
     }
   }
-  T acc = O::zero();
+  using W = AccOps<O>;
+  using AO = typename W::A;
+  typename AO::T acc = AO::zero();
   if (!on_diag || ibeg >= iend) return acc;
   // inner odometer start
   uint32_t ival[WALK_MAXL];
@@ -389,9 +391,9 @@ __device__ __forceinline__ typename O::T walk_point(const WalkDesc& d, int64_t y
     for (int k = 0; k < d.nin; ++k) off[k] += (int64_t)ival[j] * d.stride[sbase + j * d.nin + k];
   }
   for (int64_t ip = ibeg; ip < iend; ++ip) {
-    T prod = ((const T*)d.x[0])[off[0]];
-    for (int k = 1; k < d.nin; ++k) prod = O::mul(prod, ((const T*)d.x[k])[off[k]]);
-    acc = O::add(acc, prod);
+    typename AO::T prod = W::widen(((const T*)d.x[0])[off[0]]);
+    for (int k = 1; k < d.nin; ++k) prod = AO::mul(prod, W::widen(((const T*)d.x[k])[off[k]]));
+    acc = AO::add(acc, prod);
     // advance the odometer
     for (int j = 0; j < d.ninner; ++j) {
       const int32_t* sj = d.stride + sbase + j * d.nin;
@@ -420,7 +422,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkDesc d) {
     int64_t iend = ibeg + d.chunk;
     if (iend > d.ninner_pts) iend = d.ninner_pts;
     bool on_diag;
-    T acc = walk_point<O>(d, yi, ibeg, iend, on_diag);
+    const T acc = AccOps<O>::narrow(walk_point<O>(d, yi, ibeg, iend, on_diag));
     if (d.nsplit == 1) {
       y[yi] = epilogue<O>(d.mode, sx, sy, acc, y + yi);
     } else {
@@ -438,9 +440,10 @@ __global__ __launch_bounds__(256) void k_walk_reduce(WalkDesc d) {
   const T* part = (const T*)d.partial;
   for (int64_t yi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; yi < d.ny;
        yi += (int64_t)gridDim.x * blockDim.x) {
-    T acc = O::zero();
-    for (int s = 0; s < d.nsplit; ++s) acc = O::add(acc, part[(int64_t)s * d.ny + yi]);
-    y[yi] = epilogue<O>(d.mode, sx, sy, acc, y + yi);
+    using W = AccOps<O>;
+    typename W::A::T acc = W::A::zero();
+    for (int s = 0; s < d.nsplit; ++s) acc = W::A::add(acc, W::widen(part[(int64_t)s * d.ny + yi]));
+    y[yi] = epilogue<O>(d.mode, sx, sy, W::narrow(acc), y + yi);
   }
 }
 

@@ -35,11 +35,14 @@ __global__ __launch_bounds__(256) void k_gemm_tile(GemmDesc d, int64_t tiles_m, 
   const bool b_kfast = (d.sbk == 1) || (d.N == 1);
   const T* Ab = A + batch * d.sab;
   const T* Bb = B + batch * d.sbb;
-  T acc[4][4];
+  // Float32 / ComplexF32 products and sums in FP64 (AccOps), rounded once per output
+  using W = AccOps<O>;
+  using AO = typename W::A;
+  typename AO::T acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = O::zero();
+    for (int j = 0; j < 4; ++j) acc[i][j] = AO::zero();
   for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
     // all eight loads of the step are issued before any is staged: lanes outside the operand
     // load its first element and stage a zero
@@ -69,26 +72,26 @@ __global__ __launch_bounds__(256) void k_gemm_tile(GemmDesc d, int64_t tiles_m, 
     __syncthreads();
     const int kn = (int)((kend - k0) < BK ? (kend - k0) : BK);
     for (int kk = 0; kk < kn; ++kk) {
-      T a[4], b[4];
+      typename AO::T a[4], b[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk][tm + 16 * i];
+      for (int i = 0; i < 4; ++i) a[i] = W::widen(As[kk][tm + 16 * i]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tn + 16 * j];
+      for (int j = 0; j < 4; ++j) b[j] = W::widen(Bs[kk][tn + 16 * j]);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = O::fma(a[i], b[j], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = AO::fma(a[i], b[j], acc[i][j]);
     }
     __syncthreads();
   }
   if (d.ksplit > 1) {
-    T* W = (T*)d.work + ((ks * d.B + batch) * d.M) * d.N;
+    T* Wp = (T*)d.work + ((ks * d.B + batch) * d.M) * d.N;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         int64_t m = m0 + tm + 16 * i, n = n0 + tn + 16 * j;
-        if (m < d.M && n < d.N) W[m + n * d.M] = acc[i][j];
+        if (m < d.M && n < d.N) Wp[m + n * d.M] = W::narrow(acc[i][j]);
       }
     return;
   }
@@ -101,7 +104,7 @@ __global__ __launch_bounds__(256) void k_gemm_tile(GemmDesc d, int64_t tiles_m, 
       int64_t m = m0 + tm + 16 * i, n = n0 + tn + 16 * j;
       if (m < d.M && n < d.N) {
         T* yp = Y + m * d.sym + n * d.syn;
-        *yp = epilogue<O>(d.mode, sx, sy, acc[i][j], yp);
+        *yp = epilogue<O>(d.mode, sx, sy, W::narrow(acc[i][j]), yp);
       }
     }
 }
@@ -117,19 +120,22 @@ __global__ __launch_bounds__(256) void k_gemm_splitk_reduce(GemmDesc d) {
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t batch = e / per, r = e - batch * per;
     const int64_t n = r / d.M, m = r - n * d.M;
-    // partials summed in split order; loaded 16 at a time so their latencies overlap
-    T acc = O::zero();
+    // partials summed in split order in the accumulation arithmetic (FP64 for Float32 and
+    // ComplexF32), loaded 16 at a time so their latencies overlap
+    using Wd = AccOps<O>;
+    using A = typename Wd::A;
+    typename A::T acc = A::zero();
     int s = 0;
     for (; s + 16 <= d.ksplit; s += 16) {
       T v[16];
 #pragma unroll
       for (int q = 0; q < 16; ++q) v[q] = W[(int64_t)(s + q) * total + e];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) acc = O::add(acc, v[q]);
+      for (int q = 0; q < 16; ++q) acc = A::add(acc, Wd::widen(v[q]));
     }
-    for (; s < d.ksplit; ++s) acc = O::add(acc, W[(int64_t)s * total + e]);
+    for (; s < d.ksplit; ++s) acc = A::add(acc, Wd::widen(W[(int64_t)s * total + e]));
     T* yp = (T*)d.y + batch * d.syb + m * d.sym + n * d.syn;
-    *yp = epilogue<O>(d.mode, sx, sy, acc, yp);
+    *yp = epilogue<O>(d.mode, sx, sy, Wd::narrow(acc), yp);
   }
 }
 
@@ -166,10 +172,12 @@ __global__ __launch_bounds__(256) void k_gemm_strided(StridedGemmDesc d) {
       py += v * d.oy[q];
       r = qq;
     }
-    T acc = O::zero();
-    for (int k = 0; k < d.nk; ++k) acc = O::fma(A[pa + ska[k]], B[pb + skb[k]], acc);
+    using W = AccOps<O>;
+    typename W::A::T acc = W::A::zero();
+    for (int k = 0; k < d.nk; ++k)
+      acc = W::A::fma(W::widen(A[pa + ska[k]]), W::widen(B[pb + skb[k]]), acc);
     T* yp = Y + py;
-    *yp = epilogue<O>(d.mode, sx, sy, acc, yp);
+    *yp = epilogue<O>(d.mode, sx, sy, W::narrow(acc), yp);
   }
 }
 
@@ -211,9 +219,11 @@ __global__ __launch_bounds__(SF_THREADS) void k_fused_steps(const FusedStep* __r
         py += v * d.oy[q];
         r = qq;
       }
-      T acc = O::zero();
-      for (int k = 0; k < nk; ++k) acc = O::fma(A[pa + ska[k]], B[pb + skb[k]], acc);
-      Y[py] = acc;
+      using W = AccOps<O>;
+      typename W::A::T acc = W::A::zero();
+      for (int k = 0; k < nk; ++k)
+        acc = W::A::fma(W::widen(A[pa + ska[k]]), W::widen(B[pb + skb[k]]), acc);
+      Y[py] = W::narrow(acc);
     }
     // this step's outputs are the next steps' operands; the offset table is rewritten next
     __syncthreads();
@@ -236,21 +246,28 @@ __global__ __launch_bounds__(256) void k_gemm_direct(GemmDesc d, int order_m_fas
     if (order_m_fast) { n = r / d.M; m = r - n * d.M; } else { m = r / d.N; n = r - m * d.N; }
     const T* ap = A + batch * d.sab + m * d.sam;
     const T* bp = B + batch * d.sbb + n * d.sbn;
-    T acc = O::zero();
-    for (int64_t k = 0; k < d.K; ++k) acc = O::fma(ap[k * d.sak], bp[k * d.sbk], acc);
+    using W = AccOps<O>;
+    typename W::A::T acc = W::A::zero();
+    for (int64_t k = 0; k < d.K; ++k)
+      acc = W::A::fma(W::widen(ap[k * d.sak]), W::widen(bp[k * d.sbk]), acc);
     T* yp = (T*)d.y + batch * d.syb + m * d.sym + n * d.syn;
-    *yp = epilogue<O>(d.mode, sx, sy, acc, yp);
+    *yp = epilogue<O>(d.mode, sx, sy, W::narrow(acc), yp);
   }
 }
 
 // Few outputs and a long K (the last steps of a network contracted to a scalar: inner
 // products of two large tensors).  Block (output e, part p) of 256 threads sums k over its
 // chunk, strided by the block size, then reduces in LDS in a fixed tree order; with parts > 1
-// the partials go to work[p][e] and k_gemm_splitk_reduce finishes in part order.
+// the partials go to work[p][e] and k_dot_reduce finishes in part order.  Float32 and
+// ComplexF32 products and sums run in FP64 (AccOps): the kernel is bound by HBM, and these
+// sums (the final scalar of a network) are the deepest of a contraction.
 template <class O>
 __global__ __launch_bounds__(256) void k_gemm_dot(GemmDesc d, int64_t kchunk) {
   using T = typename O::T;
-  __shared__ T red[256];
+  using W = AccOps<O>;
+  using A = typename W::A;
+  using AT = typename A::T;
+  __shared__ AT red[256];
   const int t = threadIdx.x;
   const int64_t e = blockIdx.x, part = blockIdx.y;
   const int64_t per = d.M * d.N;
@@ -260,22 +277,50 @@ __global__ __launch_bounds__(256) void k_gemm_dot(GemmDesc d, int64_t kchunk) {
   const T* __restrict__ bp = (const T*)d.b + batch * d.sbb + n * d.sbn;
   const int64_t k0 = part * kchunk;
   const int64_t k1 = (k0 + kchunk < d.K) ? k0 + kchunk : d.K;
-  T acc = O::zero();
-  for (int64_t k = k0 + t; k < k1; k += 256) acc = O::fma(ap[k * d.sak], bp[k * d.sbk], acc);
+  AT acc = A::zero();
+  for (int64_t k = k0 + t; k < k1; k += 256)
+    acc = A::fma(W::widen(ap[k * d.sak]), W::widen(bp[k * d.sbk]), acc);
   red[t] = acc;
   __syncthreads();
 #pragma unroll
   for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) red[t] = O::add(red[t], red[t + s]);
+    if (t < s) red[t] = A::add(red[t], red[t + s]);
     __syncthreads();
   }
   if (t != 0) return;
   if (d.ksplit > 1) {
-    ((T*)d.work)[part * per * d.B + e] = red[0];
+    ((AT*)d.work)[part * per * d.B + e] = red[0];
     return;
   }
   T* yp = (T*)d.y + batch * d.syb + m * d.sym + n * d.syn;
-  *yp = epilogue<O>(d.mode, O::from(d.sx), O::from(d.sy), red[0], yp);
+  *yp = epilogue<O>(d.mode, O::from(d.sx), O::from(d.sy), W::narrow(red[0]), yp);
+}
+
+// Sum the dot kernel's partials (accumulation type) in part order, round once, epilogue.
+template <class O>
+__global__ __launch_bounds__(256) void k_dot_reduce(GemmDesc d) {
+  using T = typename O::T;
+  using W = AccOps<O>;
+  using A = typename W::A;
+  using AT = typename A::T;
+  const T sx = O::from(d.sx), sy = O::from(d.sy);
+  const int64_t per = d.M * d.N, total = per * d.B;
+  const AT* Wk = (const AT*)d.work;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t batch = e / per, r = e - batch * per;
+    const int64_t n = r / d.M, m = r - n * d.M;
+    AT acc = A::zero();
+    for (int s = 0; s < d.ksplit; ++s) acc = A::add(acc, Wk[(int64_t)s * total + e]);
+    T* yp = (T*)d.y + batch * d.syb + m * d.sym + n * d.syn;
+    *yp = epilogue<O>(d.mode, sx, sy, W::narrow(acc), yp);
+  }
+}
+
+int64_t dot_workspace(int dt, const GemmDesc& d, int parts) {
+  if (parts <= 1) return 0;
+  const int64_t es = (dt == F32 || dt == C32) ? 2 * (int64_t)elem_size(dt) : (int64_t)elem_size(dt);
+  return (int64_t)parts * d.M * d.N * d.B * es;
 }
 
 int dot_plan_parts(const GemmDesc& d) {
@@ -304,7 +349,7 @@ void launch_gemm_dot(int dt, GemmDesc d, int parts, hipStream_t st) {
     }
     if (parts > 1) {
       TimedScope ts(KC_OTHER, st);
-      hipLaunchKernelGGL(k_gemm_splitk_reduce<O>, dim3(grid_cap(outputs)), dim3(256), 0, st, d);
+      hipLaunchKernelGGL(k_dot_reduce<O>, dim3(grid_cap(outputs)), dim3(256), 0, st, d);
     }
   });
 }

@@ -239,6 +239,34 @@ __global__ __launch_bounds__(256) void k_mfma_bf16x3(GemmDesc d, int64_t tiles_m
 }
 
 // ---- FP32 ------------------------------------------------------------------------------------
+// Two-level accumulation for long K: an MFMA accumulator sums its products in order, so its
+// FP32 rounding error grows as sqrt(depth) (about u*sqrt(d/6) relative, 1.6e-6 at d = 4096).
+// With FOLD, every FOLD_DEPTH real products the accumulators are added into running sums
+// and restart from zero, so the error is about u*(sqrt(FOLD_DEPTH/6) + sqrt(K/FOLD_DEPTH/6)):
+// 6e-7 at K = 2^16 instead of 6e-6.  The adds are 64 VALU operations per lane per 256-deep
+// stretch of MFMA work.
+constexpr int FOLD_DEPTH = 256;
+
+// init: tot = 0.  Otherwise tot += acc and acc = 0; last: acc = tot + acc (the final sum).
+__device__ __forceinline__ void fold_acc(floatx16 (&tot)[2][2], floatx16 (&acc)[2][2], bool init,
+                                         bool last = false) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        if (init) {
+          tot[i][j][r] = 0.f;
+        } else if (last) {
+          acc[i][j][r] = tot[i][j][r] + acc[i][j][r];
+        } else {
+          tot[i][j][r] += acc[i][j][r];
+          acc[i][j][r] = 0.f;
+        }
+      }
+}
+
 constexpr int F32_BK = 32;
 constexpr int F32_LDK = F32_BK + 4;
 
@@ -269,8 +297,8 @@ __device__ __forceinline__ void store_f32(float (*__restrict__ s)[F32_LDK], cons
   }
 }
 
-template <bool AK, bool BK>
-__global__ __launch_bounds__(256) void k_mfma_f32(GemmDesc d, int64_t tiles_m, int64_t tiles_n,
+template <bool AK, bool BK, bool FOLD>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FOLD ? 2 : 1, FOLD ? 2 : 8))) void k_mfma_f32(GemmDesc d, int64_t tiles_m, int64_t tiles_n,
                                                   int64_t kchunk, int vec_a, int vec_b) {
   __shared__ __attribute__((aligned(16))) float sA[128][F32_LDK];
   __shared__ __attribute__((aligned(16))) float sB[128][F32_LDK];
@@ -289,6 +317,9 @@ __global__ __launch_bounds__(256) void k_mfma_f32(GemmDesc d, int64_t tiles_m, i
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  floatx16 tot[2][2];  // FOLD: running sums the MFMA accumulators fold into (see fold_acc)
+  if constexpr (FOLD) fold_acc(tot, acc, true);
+  int since_fold = 0;
   float ra[F32_BK / 2], rb[F32_BK / 2];
   load_f32_tile<AK, F32_BK>(A, m0, kbeg, d.M, kend, d.sam, d.sak, vec_a, ra);
   load_f32_tile<BK, F32_BK>(B, n0, kbeg, d.N, kend, d.sbn, d.sbk, vec_b, rb);
@@ -319,8 +350,15 @@ __global__ __launch_bounds__(256) void k_mfma_f32(GemmDesc d, int64_t tiles_m, i
           acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[nb].w, bf[mb].w, acc[nb][mb], 0, 0, 0);
         }
     }
+    if constexpr (FOLD) {
+      if (++since_fold == FOLD_DEPTH / F32_BK) {
+        since_fold = 0;
+        fold_acc(tot, acc, false);
+      }
+    }
     __syncthreads();
   }
+  if constexpr (FOLD) fold_acc(tot, acc, false, true);
   store_acc32<OpsF32>(d, acc, m0 + wm, n0 + wn, ti.batch, ti.ks, lane);
 }
 
@@ -370,8 +408,8 @@ __device__ __forceinline__ void load_c32_b(const c32* __restrict__ P, int64_t r0
   }
 }
 
-template <bool AK, bool BK>
-__global__ __launch_bounds__(256) void k_mfma_c32(GemmDesc d, int64_t tiles_m, int64_t tiles_n,
+template <bool AK, bool BK, bool FOLD>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FOLD ? 2 : 1, FOLD ? 2 : 8))) void k_mfma_c32(GemmDesc d, int64_t tiles_m, int64_t tiles_n,
                                                   int64_t kchunk) {
   __shared__ __attribute__((aligned(16))) float sA[128][C32_LDK];
   __shared__ __attribute__((aligned(16))) float sB[128][C32_LDK];
@@ -390,6 +428,9 @@ __global__ __launch_bounds__(256) void k_mfma_c32(GemmDesc d, int64_t tiles_m, i
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  floatx16 tot[2][2];  // FOLD: running sums the MFMA accumulators fold into (see fold_acc)
+  if constexpr (FOLD) fold_acc(tot, acc, true);
+  int since_fold = 0;
   c32 ra[4], rb[2];
   load_c32_a<AK>(A, m0, kbeg, d.M, kend, d.sam, d.sak, ra);
   load_c32_b<BK>(B, n0, kbeg, d.N, kend, d.sbn, d.sbk, rb);
@@ -443,8 +484,15 @@ __global__ __launch_bounds__(256) void k_mfma_c32(GemmDesc d, int64_t tiles_m, i
           acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[nb].w, bf[mb].w, acc[nb][mb], 0, 0, 0);
         }
     }
+    if constexpr (FOLD) {
+      if (++since_fold == FOLD_DEPTH / (2 * C32_BK)) {
+        since_fold = 0;
+        fold_acc(tot, acc, false);
+      }
+    }
     __syncthreads();
   }
+  if constexpr (FOLD) fold_acc(tot, acc, false, true);
   // lane holds m = m0 + wm + mb*32 + (lane & 31) and complex n = n0 + wn/2 + 8*(r/4) +
   // 4*(lane/32) + r%4, with the real part in block nb = 0 and the imaginary part in nb = 1
   const c32 sx = OpsC32::from(d.sx), sy = OpsC32::from(d.sy);
@@ -617,23 +665,44 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(GemmDesc d) {
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t batch = e / per, r = e - batch * per;
     const int64_t n = r / d.M, m = r - n * d.M;
-    // partials summed in split order; loaded 16 at a time so their latencies overlap
-    T acc = O::zero();
+    // partials summed in split order in the accumulation arithmetic (FP64 for Float32 and
+    // ComplexF32), loaded 16 at a time so their latencies overlap
+    using Wd = AccOps<O>;
+    using A = typename Wd::A;
+    typename A::T acc = A::zero();
     int s = 0;
     for (; s + 16 <= d.ksplit; s += 16) {
       T v[16];
 #pragma unroll
       for (int q = 0; q < 16; ++q) v[q] = W[(int64_t)(s + q) * total + e];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) acc = O::add(acc, v[q]);
+      for (int q = 0; q < 16; ++q) acc = A::add(acc, Wd::widen(v[q]));
     }
-    for (; s < d.ksplit; ++s) acc = O::add(acc, W[(int64_t)s * total + e]);
+    for (; s < d.ksplit; ++s) acc = A::add(acc, Wd::widen(W[(int64_t)s * total + e]));
     T* yp = (T*)d.y + batch * d.syb + m * d.sym + n * d.syn;
-    *yp = epilogue<O>(d.mode, sx, sy, acc, yp);
+    *yp = epilogue<O>(d.mode, sx, sy, Wd::narrow(acc), yp);
   }
 }
 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+template <bool FOLD>
+static void launch_f32(dim3 grid, dim3 block, hipStream_t st, const GemmDesc& d, int64_t tiles_m,
+                       int64_t tiles_n, int64_t kchunk, int va, int vb, bool ak, bool bk) {
+  if (ak && bk) hipLaunchKernelGGL((k_mfma_f32<true, true, FOLD>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk, va, vb);
+  else if (ak) hipLaunchKernelGGL((k_mfma_f32<true, false, FOLD>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk, va, vb);
+  else if (bk) hipLaunchKernelGGL((k_mfma_f32<false, true, FOLD>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk, va, vb);
+  else hipLaunchKernelGGL((k_mfma_f32<false, false, FOLD>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk, va, vb);
+}
+
+template <bool FOLD>
+static void launch_c32(dim3 grid, dim3 block, hipStream_t st, const GemmDesc& d, int64_t tiles_m,
+                       int64_t tiles_n, int64_t kchunk, bool ak, bool bk) {
+  if (ak && bk) hipLaunchKernelGGL((k_mfma_c32<true, true, FOLD>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk);
+  else if (ak) hipLaunchKernelGGL((k_mfma_c32<true, false, FOLD>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk);
+  else if (bk) hipLaunchKernelGGL((k_mfma_c32<false, true, FOLD>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk);
+  else hipLaunchKernelGGL((k_mfma_c32<false, false, FOLD>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk);
+}
 
 int mfma_plan_ksplit(const GemmDesc& d, int requested) {
   if (requested > 0) return requested;
@@ -686,7 +755,8 @@ void launch_gemm_mfma(int prec, GemmDesc d, int ksplit, hipStream_t st) {
   if (prec == PREC_BF16X3) {
     OME_MFMA_LAUNCH(k_mfma_bf16x3)
   } else if (prec == PREC_FP32) {
-    OME_MFMA_LAUNCH(k_mfma_f32)
+    if (kchunk > FOLD_DEPTH) launch_f32<true>(grid, block, st, d, tiles_m, tiles_n, kchunk, va, vb, ak, bk);
+    else launch_f32<false>(grid, block, st, d, tiles_m, tiles_n, kchunk, va, vb, ak, bk);
   } else {
     OME_MFMA_LAUNCH(k_mfma_f64)
   }
@@ -733,10 +803,9 @@ void launch_gemm_mfma_c32(GemmDesc d, int ksplit, hipStream_t st) {
   const dim3 grid((uint32_t)blocks), block(256);
   {
     TimedScope ts(KC_GEMM_MFMA, st);
-    if (ak && bk) hipLaunchKernelGGL((k_mfma_c32<true, true>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk);
-    else if (ak) hipLaunchKernelGGL((k_mfma_c32<true, false>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk);
-    else if (bk) hipLaunchKernelGGL((k_mfma_c32<false, true>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk);
-    else hipLaunchKernelGGL((k_mfma_c32<false, false>), grid, block, 0, st, d, tiles_m, tiles_n, kchunk);
+    // a complex k is two real products of the realified sum
+    if (2 * kchunk > FOLD_DEPTH) launch_c32<true>(grid, block, st, d, tiles_m, tiles_n, kchunk, ak, bk);
+    else launch_c32<false>(grid, block, st, d, tiles_m, tiles_n, kchunk, ak, bk);
   }
   if (ksplit > 1) {
     TimedScope ts(KC_OTHER, st);

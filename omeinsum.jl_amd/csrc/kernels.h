@@ -105,6 +105,7 @@ void launch_fused_steps(int dt, const FusedStep* steps, const int32_t* first, in
 
 // Few outputs (M*N*batch <= 1024), long K: one workgroup per output and K part.
 int dot_plan_parts(const GemmDesc& d);
+int64_t dot_workspace(int dt, const GemmDesc& d, int parts);
 void launch_gemm_dot(int dt, GemmDesc d, int parts, hipStream_t st);
 
 // GEMMs on the real matrix cores in another arithmetic than the element type: complex

@@ -369,7 +369,7 @@ Status run_gemm(ome_ctx* ctx, int dt, GemmDesc d, hipStream_t st, int precision)
     const int parts = dot_plan_parts(d);
     Temp ws;
     if (parts > 1) {
-      ws = Temp(ctx, (size_t)parts * d.M * d.N * d.B * elem_size(dt));
+      ws = Temp(ctx, (size_t)dot_workspace(dt, d, parts));
       if (!ws.p) return Status::err(OME_ERR_OOM, "device pool exhausted (split-K workspace)");
       d.work = ws.p;
     }

@@ -82,3 +82,19 @@ def gaussian_tensors(code, size_dict, seed, dtype=np.complex64):
             x = rng.normal(0, 1, shape)
         xs.append(np.asarray(x, dtype=dtype))
     return xs
+
+
+def request_stream(nreq=48, seed=5000):
+    """The serving workload's request list: request q is a random 3-regular graph on an even
+    number of vertices n uniform in [40, 140], with its own graph seed and tensor seed, all
+    drawn from one PCG64 stream in the order n, graph seed, tensor seed.  Returns dicts with
+    keys n, graph_seed, tensor_seed, edges."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(nreq):
+        n = 2 * int(rng.integers(20, 71))
+        gseed = int(rng.integers(1 << 30))
+        tseed = int(rng.integers(1 << 30))
+        out.append({"n": n, "graph_seed": gseed, "tensor_seed": tseed,
+                    "edges": random_regular_graph(n, 3, seed=gseed)})
+    return out

@@ -9,6 +9,13 @@ exact Int64 / max-plus, and the numpy DefaultBackend restatement for the tree):
   network_100.json                       the 100-vertex 3-regular graph (seed 2000), its greedy
                                          contraction tree (writejson format) and the FP64 value
                                          of the full contraction of its seeded ComplexF32 tensors
+  request_stream_48.json                 the serving workload's 48 requests (networks.request_
+                                         stream, seed 5000): per request the vertex count, graph
+                                         and tensor seeds, the greedy tree's complexity and flop
+                                         count, and the FP64 value of its full contraction
+
+``python tests/golden/make_fixtures.py --stream`` writes only the request-stream fixture (the
+FP64 contraction of all 48 networks, 5.2e12 flop, takes minutes on the host).
 """
 import json
 import os
@@ -100,5 +107,40 @@ def main():
                    "complexity": list(co.contraction_complexity(tree, sd))}, f)
 
 
+STREAM_SLICE_SC = 24   # host memory bound for the FP64 contraction: 2^24 complex128 = 256 MiB
+
+
+def stream_request_value(q):
+    """FP64 value of one request's full contraction through the oracle (complex128): its
+    greedy tree, sliced to 2^STREAM_SLICE_SC elements when larger, cross-slice sum in FP64."""
+    code = networks.network_code(q["n"], q["edges"])
+    sd = {e: 2 for e in range(len(q["edges"]))}
+    tree = co.optimize_code(code, sd)
+    xs = [x.astype(np.complex128) for x in
+          networks.gaussian_tensors(code, sd, q["tensor_seed"], np.complex64)]
+    tc, sc, _ = co.contraction_complexity(tree, sd)
+    if sc > STREAM_SLICE_SC:
+        st = co.slice_code(tree, sd, sc_target=STREAM_SLICE_SC)
+        val = db.sliced_einsum(st.slicing, st.eins.to_plain(), xs, sd)
+    else:
+        val = db.nested_einsum(tree.to_plain(), xs, sd)
+    return tree, sd, [tc, sc], complex(np.asarray(val)[()])
+
+
+def stream_fixture(path=os.path.join(HERE, "request_stream_48.json"), nreq=48, seed=5000):
+    out = []
+    for q in networks.request_stream(nreq, seed):
+        tree, sd, cx, val = stream_request_value(q)
+        out.append({"n": q["n"], "graph_seed": q["graph_seed"], "tensor_seed": q["tensor_seed"],
+                    "complexity": cx, "flop": co.flop(tree, sd),
+                    "value_fp64": [val.real, val.imag]})
+        print("request %d: n %d, sc %d, value %r" % (len(out) - 1, q["n"], cx[1], val), flush=True)
+    with open(path, "w") as f:
+        json.dump({"seed": seed, "requests": out}, f, indent=0)
+
+
 if __name__ == "__main__":
-    main()
+    if "--stream" in sys.argv:
+        stream_fixture()
+    else:
+        main()

@@ -1,4 +1,7 @@
 """Shared helpers for the parity tests: seeded inputs, device calls, oracle comparison."""
+import json
+import os
+
 import numpy as np
 
 F_TOL = {np.float32: 2e-5, np.complex64: 2e-5, np.float64: 1e-12, np.complex128: 1e-12}
@@ -30,9 +33,45 @@ def wide(dtype):
         np.dtype(dtype), dtype)
 
 
-def assert_close(got, ref64, dtype, k_terms=1, what=""):
+# Norm-wise bound ||got - ref||_F / ||ref||_F for Float32 / ComplexF32 results, by the
+# arithmetic the product ran in (the ``precision`` argument of assert_close):
+#   fp32    FP32 operands and FP32 accumulation: the accumulation error of a depth-d sum of
+#           random-sign products is about u*sqrt(d/6) relative (u = 2^-24): 1.6e-6 at d = 4096.
+#           Bound 2e-6, growing as sqrt(d) beyond d = 4096.
+#   3m_*    Gauss's 3-product complex form: the imaginary part is (ar+ai)(br+bi) - ar*br - ai*bi,
+#           whose terms are up to twice the result's size: 3x the fp32 bound.
+#   bf16x3  3-term BF16 split: each operand keeps ~16 bits (x - hi - lo ~ 2^-17 |x|), so every
+#           product errs by ~2^-16.5 relative whatever the depth; 1e-5 up to d = 65536.  A split
+#           that drops one correction product errs by ~1e-3 and fails (tests/test_tolerance.py).
+#   fp64    FP64 products rounded once to Float32: the fp32 bound.
+# d counts the real products summed into one output component: K for a real product, 2K for a
+# complex one (the realified GEMM's depth).
+NORM_TOL = {"fp32": 2e-6, "fp64": 2e-6, "3m_fp32": 6e-6, "3m_fp64": 6e-6, "bf16x3": 1e-5,
+            "3m_bf16x3": 3e-5}
+
+
+def norm_bound(precision, depth, complex_=False):
+    d = max(1.0, float(depth)) * (2.0 if complex_ else 1.0)
+    if precision.endswith("bf16x3"):
+        return NORM_TOL[precision] * max(1.0, np.sqrt(d / 65536.0))
+    return NORM_TOL[precision] * max(1.0, np.sqrt(d / 4096.0))
+
+
+def _log_error(what, dtype, precision, depth, err, bound):
+    path = os.environ.get("OME_TEST_ERRLOG")
+    if path:
+        with open(path, "a") as f:
+            f.write(json.dumps({"test": os.environ.get("PYTEST_CURRENT_TEST", "?"), "what": str(what),
+                                "dtype": np.dtype(dtype).name, "precision": precision,
+                                "depth": float(depth), "rel_err": err, "bound": bound}) + "\n")
+
+
+def assert_close(got, ref64, dtype, k_terms=1, what="", precision="fp32", depth=None, norm=True):
     """Floating point: |got - ref| <= tol * sqrt(k) * (max|ref| + 1) elementwise, with the
-    tolerance written per type in F_TOL; integer/tropical: bit-exact."""
+    tolerance written per type in F_TOL; for Float32 / ComplexF32 also the norm-wise bound
+    ||got - ref||_F / ||ref||_F <= norm_bound(precision, depth) (depth defaults to k_terms;
+    norm=False leaves it out for a result whose size is set by cancellation, such as a sum of
+    random-sign terms to one scalar); integer/tropical: bit-exact."""
     got = np.asarray(got)
     ref64 = np.asarray(ref64)
     assert got.shape == ref64.shape, (what, got.shape, ref64.shape)
@@ -44,6 +83,12 @@ def assert_close(got, ref64, dtype, k_terms=1, what=""):
     scale = np.max(np.abs(ref64)) if ref64.size else 0.0
     err = np.max(np.abs(got.astype(wide(dt)) - ref64)) if ref64.size else 0.0
     assert err <= tol * (scale + 1.0), "%s: max err %.3e > %.3e (scale %.3e)" % (what, err, tol * (scale + 1), scale)
+    if norm and dt.type in (np.float32, np.complex64) and ref64.size and np.any(ref64 != 0):
+        d = k_terms if depth is None else depth
+        e = rel_err(got, ref64)
+        bound = norm_bound(precision, d, dt.kind == "c")
+        _log_error(what, dt, precision, d, e, bound)
+        assert e <= bound, "%s: norm-wise error %.3e > %.3e (%s, depth %g)" % (what, e, bound, precision, d)
 
 
 def rel_err(got, ref64):

@@ -76,11 +76,13 @@ def test_complex_as_real_gemm(spec, algo, dtype, prec):
     ref = oracle(ixs, iy, xs, SIZES)
     k = SIZES["j"]
     # 3M forms (ar+ai)(br+bi) - ar*br - ai*bi: operands up to twice as large, three terms
-    assert_close(got, ref, dtype, 4 * k * (3 if algo == "3m" else 1), spec)
+    nprec = ("3m_" if algo == "3m" else "") + prec
+    assert_close(got, ref, dtype, 4 * k * (3 if algo == "3m" else 1), spec, precision=nprec, depth=k)
     sx, sy = 1.5 - 0.5j, 0.25 + 1j
     got = run(ixs, iy, xs, SIZES, be, y0=y0, sx=sx, sy=sy)
     ref = oracle(ixs, iy, xs, SIZES, y0=y0, sx=sx, sy=sy)
-    assert_close(got, ref, dtype, 16 * k * (3 if algo == "3m" else 1), spec + " scaled")
+    assert_close(got, ref, dtype, 16 * k * (3 if algo == "3m" else 1), spec + " scaled",
+                 precision=nprec, depth=k)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
@@ -95,11 +97,13 @@ def test_complex_split_k_and_edges(prec):
         ixs, iy = parse(spec)
         xs = [rand(r, [sd[l] for l in ix], np.complex64) for ix in ixs]
         got = run(ixs, iy, xs, sd, be)
-        assert_close(got, oracle(ixs, iy, xs, sd), np.complex64, 4 * sd["j"], spec)
+        assert_close(got, oracle(ixs, iy, xs, sd), np.complex64, 4 * sd["j"], spec,
+                     precision=prec, depth=sd["j"])
         y0 = rand(r, [sd[l] for l in iy], np.complex64)
         got = run(ixs, iy, xs, sd, be, y0=y0, sx=0.5 + 2j, sy=-1 + 0.5j)
         ref = oracle(ixs, iy, xs, sd, y0=y0, sx=0.5 + 2j, sy=-1 + 0.5j)
-        assert_close(got, ref, np.complex64, 4 * sd["j"] + 8, spec + " scaled")
+        assert_close(got, ref, np.complex64, 4 * sd["j"] + 8, spec + " scaled",
+                     precision=prec, depth=sd["j"])
 
 
 def scattered(r, m_sizes, k_sizes, n_sizes):
@@ -131,11 +135,13 @@ def test_complex_scattered_labels(prec):
         xs = [rand(r, [sd[l] for l in ix], np.complex64) for ix in ixs]
         k = int(np.prod(ks))
         got = run(ixs, iy, xs, sd, be)
-        assert_close(got, oracle(ixs, iy, xs, sd), np.complex64, 4 * k, str((ms, ks, ns)))
+        assert_close(got, oracle(ixs, iy, xs, sd), np.complex64, 4 * k, str((ms, ks, ns)),
+                     precision=prec, depth=k)
         y0 = rand(r, [sd[l] for l in iy], np.complex64)
         got = run(ixs, iy, xs, sd, be, y0=y0, sx=0.5 - 1j, sy=2 + 0.25j)
         ref = oracle(ixs, iy, xs, sd, y0=y0, sx=0.5 - 1j, sy=2 + 0.25j)
-        assert_close(got, ref, np.complex64, 4 * k + 8, str((ms, ks, ns)) + " scaled")
+        assert_close(got, ref, np.complex64, 4 * k + 8, str((ms, ks, ns)) + " scaled",
+                     precision=prec, depth=k)
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.complex64])
@@ -191,7 +197,7 @@ def test_few_outputs_long_k(spec, sd, dtype):
             if dt == np.int64:
                 assert np.array_equal(got, ref), spec
             else:
-                assert_close(got, ref, dt, 4 * k, spec + " scaled")
+                assert_close(got, ref, dt, 4 * k, spec + " scaled", depth=k)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
@@ -219,7 +225,8 @@ def test_c4_shaped_backward_rule(prec, dtype):
     W = wide(dtype)
     for i in range(2):
         ref = db.einsum_grad(ixs, [x.astype(W) for x in xs], iy, sd, dy.astype(W), i)
-        assert_close(got[i], ref, dtype, 16 * 4096, "grad %d" % i)
+        # each gradient contracts two labels of size 16: depth 256
+        assert_close(got[i], ref, dtype, 16 * 4096, "grad %d" % i, precision=prec, depth=256)
 
 
 def test_c4_shaped_gradient_finite_differences():

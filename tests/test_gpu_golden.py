@@ -43,7 +43,10 @@ def test_golden_einsum_cases():
                 assert np.array_equal(got, ref), (backend, case["spec"])
             else:
                 k = int(np.prod([case["sizes"][l] for l in set(l for ix in ixs for l in ix) - set(iy)] or [1]))
-                assert_close(got, ref, np.dtype(case["dtype"]), k, case["spec"])
+                # a sum to one scalar is judged elementwise only: its norm-wise error is
+                # the cancellation of the random-sign terms, not the arithmetic's
+                assert_close(got, ref, np.dtype(case["dtype"]), k, case["spec"],
+                             norm=ref.size > 1)
 
 
 def test_golden_network_value():
