@@ -202,6 +202,21 @@ int ome_einsum(ome_ctx* ctx, int dtype, int nin, const void* const* xs, const in
                const int32_t* ixs, void* y, int niy, const int32_t* iy, int nlabels,
                const int64_t* sizes, const ome_scalar* sx, const ome_scalar* sy);
 
+/* No counterpart in the reference: ome_einsum for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, semantics and errors as ome_einsum; the planner's temporaries
+ * (permuted operands, split-K partials, packed BF16 operands, real-GEMM operands) come from
+ * one device array of OME_NOALLOC_SCRATCH_BYTES per GPU, held by the library until the
+ * process ends, instead of the pool, and the call allocates nothing.  A contraction whose
+ * temporaries exceed the array fails with OME_ERR_OOM before anything is launched.  Every
+ * call on one GPU uses that GPU's array: calls that overlap on a GPU (other contexts or
+ * streams of the same device) race; calls in order on one stream do not. */
+#define OME_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int ome_einsum_noalloc(ome_ctx* ctx, int dtype, int nin, const void* const* xs,
+                       const int32_t* nix, const int32_t* ixs, void* y, int niy,
+                       const int32_t* iy, int nlabels, const int64_t* sizes,
+                       const ome_scalar* sx, const ome_scalar* sy);
+
 /* loop_einsum!(ixs, iy, xs, y, sx, sy, size_dict): src/loop_einsum.jl:21-32 (and its
  * ROCArray override ext/AMDGPUExt.jl:54-86).  Always the index-walk kernel; fails with
  * OME_ERR_LOOPS_FORBIDDEN when allow_loops is off. */

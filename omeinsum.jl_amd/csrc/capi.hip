@@ -195,6 +195,41 @@ static int finish(ome_ctx* ctx, const Status& s) {
 #define OME_CHECK_CTX(ctx) \
   if (!(ctx)) return fail(nullptr, OME_ERR_INVALID_ARG, "null context")
 
+// ---- fixed scratch of ome_einsum_noalloc -----------------------------------------------------
+// One device array per GPU the library runs on (the code object's own memory, held until the
+// process ends).  In host code `scratch` names a host-side stand-in, never the device memory:
+// its device address on each GPU comes from hipGetSymbolAddress at ome_init, with that GPU
+// current, and only that address is passed to kernels.
+static __device__ __align__(256) char scratch[OME_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};        // device address of `scratch`, per device
+hipError_t g_scratch_err[kMaxDevices] = {};  // the lookup's failure, per device
+
+// Resolves `scratch` on `device` (the current device) once; later calls reuse the address.
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    hipGetLastError();
+    g_scratch_err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+    return;
+  }
+  g_scratch_ptr[device] = p;
+}
+
+void* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
 extern "C" {
 
 int ome_abi_version(void) { return OME_ABI_VERSION; }
@@ -230,6 +265,7 @@ int ome_init(int device, size_t pool_bytes, void* stream, ome_ctx** out) {
   ctx->info.lds_per_cu = (int64_t)prop.maxSharedMemoryPerMultiProcessor;
   ctx->info.clock_mhz = prop.clockRate / 1000;
   ctx->info.device = device;
+  scratch_setup(device);  // device address of ome_einsum_noalloc's scratch on this GPU
   if (pool_bytes) {  // warm the pool with one reusable block
     void* p = ctx->pool->alloc(pool_bytes);
     ctx->pool->free(p);
@@ -370,7 +406,7 @@ int ome_memcpy_d2d(ome_ctx* ctx, void* dst, const void* src, size_t bytes) {
 static int einsum_entry(ome_ctx* ctx, int dtype, int nin, const void* const* xs, const int32_t* nix,
                         const int32_t* ixs, void* y, int niy, const int32_t* iy, int nlabels,
                         const int64_t* sizes, const ome_scalar* sx, const ome_scalar* sy,
-                        bool force_loop) {
+                        bool force_loop, bool noalloc = false) {
   OME_CHECK_CTX(ctx);
   if (nin < 1 || nin > OME_MAX_INPUTS)
     return fail(ctx, OME_ERR_INVALID_ARG, "number of input tensors must be in 1..64");
@@ -393,13 +429,52 @@ static int einsum_entry(ome_ctx* ctx, int dtype, int nin, const void* const* xs,
   c.force_loop = force_loop;
   std::lock_guard<std::recursive_mutex> g(ctx->mu);
   hipSetDevice(ctx->device);
-  return finish(ctx, run_einsum(ctx, c, ctx->stream));
+  if (!noalloc) return finish(ctx, run_einsum(ctx, c, ctx->stream));
+  // ---- allocation-free: the planner's temporaries come from `scratch` on this device
+  char* base = (char*)scratch_ptr(ctx->device);
+  if (!base) {
+    const hipError_t e = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch_err[ctx->device]
+                                                                        : hipErrorInvalidDevice;
+    return fail(ctx, OME_ERR_DEVICE, std::string("ome_einsum_noalloc: the scratch array has no "
+                                                 "device address on this GPU (") +
+                                         hipGetErrorString(e) + ")");
+  }
+  // size the temporaries with a dry run of the same plan (nothing is launched)
+  Bump dry;
+  dry.base = (char*)(uintptr_t)4096;  // never dereferenced: a dry run launches nothing
+  dry.cap = (size_t)1 << 60;
+  LaunchMode& lm = launch_mode();
+  lm.dry = true;
+  ctx->bump = &dry;
+  Status r = run_einsum(ctx, c, ctx->stream);
+  ctx->bump = nullptr;
+  lm.dry = false;
+  if (!r.good()) return finish(ctx, r);
+  if (dry.high > (size_t)OME_NOALLOC_SCRATCH_BYTES)
+    return fail(ctx, OME_ERR_OOM,
+                "ome_einsum_noalloc: this contraction's temporaries need " + std::to_string(dry.high) +
+                    " bytes, above the " + std::to_string((long long)OME_NOALLOC_SCRATCH_BYTES) +
+                    "-byte scratch array (use ome_einsum)");
+  Bump b;
+  b.base = base;
+  b.cap = (size_t)OME_NOALLOC_SCRATCH_BYTES;
+  ctx->bump = &b;
+  r = run_einsum(ctx, c, ctx->stream);
+  ctx->bump = nullptr;
+  return finish(ctx, r);
 }
 
 int ome_einsum(ome_ctx* ctx, int dtype, int nin, const void* const* xs, const int32_t* nix,
                const int32_t* ixs, void* y, int niy, const int32_t* iy, int nlabels,
                const int64_t* sizes, const ome_scalar* sx, const ome_scalar* sy) {
   return einsum_entry(ctx, dtype, nin, xs, nix, ixs, y, niy, iy, nlabels, sizes, sx, sy, false);
+}
+
+int ome_einsum_noalloc(ome_ctx* ctx, int dtype, int nin, const void* const* xs, const int32_t* nix,
+                       const int32_t* ixs, void* y, int niy, const int32_t* iy, int nlabels,
+                       const int64_t* sizes, const ome_scalar* sx, const ome_scalar* sy) {
+  return einsum_entry(ctx, dtype, nin, xs, nix, ixs, y, niy, iy, nlabels, sizes, sx, sy, false,
+                      true);
 }
 
 int ome_loop_einsum(ome_ctx* ctx, int dtype, int nin, const void* const* xs, const int32_t* nix,

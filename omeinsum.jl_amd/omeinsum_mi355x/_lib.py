@@ -70,6 +70,9 @@ SIGNATURES = {
     "ome_memcpy_d2d": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "ome_einsum": (c_int, [c_void_p, c_int, c_int, P(c_void_p), P(c_i32), P(c_i32), c_void_p,
                            c_int, P(c_i32), c_int, P(c_i64), P(Scalar), P(Scalar)]),
+    "ome_einsum_noalloc": (c_int, [c_void_p, c_int, c_int, P(c_void_p), P(c_i32), P(c_i32),
+                                   c_void_p, c_int, P(c_i32), c_int, P(c_i64), P(Scalar),
+                                   P(Scalar)]),
     "ome_loop_einsum": (c_int, [c_void_p, c_int, c_int, P(c_void_p), P(c_i32), P(c_i32),
                                 c_void_p, c_int, P(c_i32), c_int, P(c_i64), P(Scalar),
                                 P(Scalar)]),
