@@ -296,6 +296,14 @@ def bench_c2(steps, warmup):
                    "arena_bytes": int(info.arena_bytes), "flops": info.flops}
     v, lg = plan.execute_scalar_normalized(dxs)
     out["normalized_rel_err_fp64"] = abs(v * np.exp(lg) - ref) / abs(ref)
+    # the normalised FP64 evaluation (inputs widened and normalised, every step in FP64):
+    # wall time per call, the value read back to the host included
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        plan.execute_scalar_normalized(dxs)
+        ts.append(time.perf_counter() - t0)
+    out["normalized_fp64_ms"] = statistics.median(ts) * 1e3
     # the CPU reference path: the oracle's per-step restatement in ComplexF32, median of up
     # to 20 runs after one warm-up (at most ~10 s)
     db.nested_einsum(tree.to_plain(), xs, sd)

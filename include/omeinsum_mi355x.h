@@ -248,7 +248,10 @@ int ome_batched_gemm(ome_ctx* ctx, int dtype, char ta, char tb, int64_t m, int64
  * dense, no transposes; dims holds (m,n,k,batch) per problem.  Float32 runs on the FP32
  * matrix cores, other types on the semiring VALU tile.  The tile shape and grid are chosen
  * per shape class by timing the candidates on first use (when beta is zero, autotuning is
- * on and the stream is not being captured) and remembered for the context. */
+ * on and the stream is not being captured) and remembered for the context.  Graph capture:
+ * a problem list (shapes and pointers) may be captured once it has run on the context; its
+ * device descriptors then stay allocated until ome_shutdown.  A list that has not run fails
+ * under capture with OME_ERR_UNSUPPORTED (the same holds for ome_grouped_permute). */
 int ome_grouped_gemm(ome_ctx* ctx, int dtype, int ngroups, const int64_t* dims,
                      const void* const* a, const void* const* b, void* const* c,
                      const ome_scalar* alpha, const ome_scalar* beta);
@@ -282,9 +285,13 @@ int ome_get_tree_info(ome_tree* tree, ome_tree_info* info);
  * rounded to the output type once. */
 int ome_tree_execute(ome_tree* tree, const void* const* xs, void* y, const ome_scalar* sx,
                      const ome_scalar* sy);
-/* Full-network contraction to a scalar with per-step log-scale normalisation: each
- * intermediate is divided by its max-abs and the logs accumulate in FP64.  Writes
- * value (re, im) and log_scale so that result = value * exp(log_scale). */
+/* No reference counterpart: full-network contraction to a scalar with log-scale
+ * normalisation.  Every input tensor is divided by its max-abs, and so is every
+ * intermediate; the logs accumulate in FP64.  Float32 / ComplexF32 trees are evaluated in
+ * FP64: each input is widened to Float64 / ComplexF64 as it is normalised, and an FP64 twin
+ * of the plan (built on first use, then kept with the plan) runs every step, the final
+ * scalar included, in FP64.  Writes value (re, im) and log_scale so that
+ * result = value * exp(log_scale). */
 int ome_tree_execute_scalar_normalized(ome_tree* tree, const void* const* xs,
                                        double* value_re_im, double* log_scale);
 /* cost_and_gradient(code, xs, ybar): src/bp.jl:122-130 (cached_einsum :33-48, back_propagate

@@ -4,8 +4,11 @@
 // Problem descriptors are built on the host, staged in pinned memory and copied to the device
 // once per distinct problem list (cache of 32 lists per context, keyed by the full list of
 // shapes and pointers), so a repeated launch over the same buffers costs one kernel launch.
-// A copy made while the stream is being captured becomes part of the graph; the pinned and
-// device buffers stay alive until the entry is evicted or the context is shut down.
+// A list used while the stream is being captured may be read by every replay of that graph
+// (its copy is a graph node, its device block a kernel argument): such an entry is pinned and
+// kept until the context is shut down.  Under capture nothing is uploaded (an upload would
+// allocate) and nothing is evicted (an eviction waits for the stream, which a capture
+// forbids): a list is captured after it has run once on the context.
 #include <algorithm>
 #include <cstring>
 #include <list>
@@ -27,6 +30,7 @@ struct GroupEntry {
   int nprob = 0;
   int64_t total = 0;
   int64_t split = 0;  // grouped permute: blocks of the shared-dim problems, which come first
+  bool pinned = false;  // used under graph capture: kept for the context's life
 };
 
 inline int ilog2(double v) {
@@ -36,6 +40,15 @@ inline int ilog2(double v) {
 }
 
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+bool capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+    hipGetLastError();
+    return true;
+  }
+  return cs != hipStreamCaptureStatusNone;
+}
 
 bool is_zero_scalar(int dt, const Scal& s) {
   bool z = false;
@@ -72,11 +85,16 @@ template <class Fill>
 static GroupEntry* group_entry(ome_ctx* ctx, hipStream_t st, const std::vector<int64_t>& key,
                                size_t bytes, Fill fill) {
   GroupState* gs = group_state(ctx);
+  const bool cap = capturing(st);
   for (auto it = gs->cache.begin(); it != gs->cache.end(); ++it)
     if (it->key == key) {
+      if (cap) it->pinned = true;
       if (it != gs->cache.begin()) gs->cache.splice(gs->cache.begin(), gs->cache, it);
       return &gs->cache.front();
     }
+  // a new list is not uploaded under capture (the upload would allocate): the caller reports
+  // that the list must run once on the context before the capture
+  if (cap) return nullptr;
   GroupEntry e;
   e.key = key;
   e.bytes = bytes;
@@ -97,12 +115,19 @@ static GroupEntry* group_entry(ome_ctx* ctx, hipStream_t st, const std::vector<i
     return nullptr;
   }
   gs->cache.push_front(std::move(e));
-  if (gs->cache.size() > 32) {
-    GroupEntry& old = gs->cache.back();
-    hipStreamSynchronize(st);
-    ctx->pool->free(old.dev);
-    hipHostFree(old.host);
-    gs->cache.pop_back();
+  if (gs->cache.size() > 32 && !cap) {
+    // evict the least recently used entry that no graph may replay
+    for (auto it = std::prev(gs->cache.end()); it != gs->cache.begin(); --it) {
+      if (it->pinned) continue;
+      if (hipStreamSynchronize(st) != hipSuccess) {  // the evicted copy may still be in flight
+        hipGetLastError();
+        break;  // keep the entry rather than free memory a kernel may still read
+      }
+      ctx->pool->free(it->dev);
+      hipHostFree(it->host);
+      gs->cache.erase(it);
+      break;
+    }
   }
   return &gs->cache.front();
 }
@@ -221,6 +246,9 @@ static int gg_grid(ome_ctx* ctx, int dt, const GGCand& c, int64_t total) {
 static Status gg_launch(ome_ctx* ctx, hipStream_t st, int dt, const GGCand& c,
                         const std::vector<GGProblem>& ps, const Scal& alpha, const Scal& beta) {
   GroupEntry* e = gg_entry(ctx, st, dt, c.variant, ps);
+  if (!e && capturing(st))
+    return Status::err(OME_ERR_UNSUPPORTED, "grouped GEMM under graph capture: run the same problem "
+                                            "list once on this context before capturing it");
   if (!e) return Status::err(OME_ERR_OOM, "no memory for the grouped GEMM descriptors");
   GGDesc g{};
   g.probs = (const GGProb*)e->dev;
@@ -233,15 +261,6 @@ static Status gg_launch(ome_ctx* ctx, hipStream_t st, int dt, const GGCand& c,
   g.mode = dt == F32 ? epi_mode<OpsF32>(alpha, beta) : 0;
   launch_grouped_gemm(dt, c.variant, g, gg_grid(ctx, dt, c, e->total), st);
   return Status::ok();
-}
-
-static bool capturing(hipStream_t st) {
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
-    hipGetLastError();
-    return true;
-  }
-  return cs != hipStreamCaptureStatusNone;
 }
 
 Status run_grouped_gemm(ome_ctx* ctx, int dt, const std::vector<GGProblem>& ps, const Scal& alpha,
@@ -570,6 +589,9 @@ Status run_grouped_permute(ome_ctx* ctx, int dt, int np, const int32_t* ndims, c
     en.total = total;
     en.split = split;
   });
+  if (!e && capturing(st))
+    return Status::err(OME_ERR_UNSUPPORTED, "grouped permute under graph capture: run the same "
+                                            "problem list once on this context before capturing it");
   if (!e) return Status::err(OME_ERR_OOM, "no memory for the grouped permute descriptors");
   GPDesc g{};
   g.probs = (const GPProb*)e->dev;

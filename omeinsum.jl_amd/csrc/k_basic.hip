@@ -464,3 +464,44 @@ void launch_walk(int dt, const WalkDesc& d, hipStream_t st) {
 }
 
 }  // namespace ome
+
+namespace ome {
+
+// y = widen(x) / scale[0]: a Float32 / ComplexF32 tensor written as Float64 / ComplexF64,
+// divided by its largest magnitude (the per-tensor normalisation of a network's inputs).
+template <class O>
+__global__ __launch_bounds__(256) void k_widen_scaled(const typename O::T* __restrict__ x,
+                                                      typename AccOps<O>::A::T* __restrict__ y,
+                                                      int64_t n, const double* __restrict__ scale) {
+  const double inv = 1.0 / scale[0];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const auto v = AccOps<O>::widen(x[i]);
+    if constexpr (O::DT == C32) {
+      y[i] = {v.x * inv, v.y * inv};
+    } else {
+      y[i] = v * inv;
+    }
+  }
+}
+
+void launch_widen_normalized(int dt, const void* x, void* y, int64_t n, double* work,
+                             double* logacc, hipStream_t st) {
+  if (n <= 0) return;
+  OME_DRY_RETURN(3);
+  const int g = grid_for(n, 256, 1024);
+  double* part = work + 1;
+  if (dt == F32) {
+    hipLaunchKernelGGL(k_maxabs_partial<OpsF32>, dim3(g), dim3(256), 0, st, (const float*)x, n, part);
+    hipLaunchKernelGGL(k_maxabs_final, dim3(1), dim3(256), 0, st, part, g, work, logacc);
+    hipLaunchKernelGGL(k_widen_scaled<OpsF32>, dim3(grid_for(n)), dim3(256), 0, st, (const float*)x,
+                       (double*)y, n, work);
+  } else if (dt == C32) {
+    hipLaunchKernelGGL(k_maxabs_partial<OpsC32>, dim3(g), dim3(256), 0, st, (const c32*)x, n, part);
+    hipLaunchKernelGGL(k_maxabs_final, dim3(1), dim3(256), 0, st, part, g, work, logacc);
+    hipLaunchKernelGGL(k_widen_scaled<OpsC32>, dim3(grid_for(n)), dim3(256), 0, st, (const c32*)x,
+                       (c64*)y, n, work);
+  }
+}
+
+}  // namespace ome

@@ -50,6 +50,10 @@ void launch_from_f64(int dt, const double* acc, void* y, int64_t n, const Scal& 
                      const Scal& sy, hipStream_t st);
 // x <- x / max|x|; logacc[0] += log(max|x|).  work needs 1 + 1024 doubles.
 void launch_normalize(int dt, void* x, int64_t n, double* work, double* logacc, hipStream_t st);
+// Float32 / ComplexF32 x -> Float64 / ComplexF64 y = x / max|x|; log(max|x|) is added to
+// logacc[0] (work: the normalisation scratch of launch_normalize)
+void launch_widen_normalized(int dt, const void* x, void* y, int64_t n, double* work,
+                             double* logacc, hipStream_t st);
 void launch_permute(int dt, const PermDesc& d, hipStream_t st);
 void launch_walk(int dt, const WalkDesc& d, hipStream_t st);
 

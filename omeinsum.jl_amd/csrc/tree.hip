@@ -135,6 +135,12 @@ struct ome_tree {
   FusedStep* fdev = nullptr;         // device descriptors, subtree by subtree
   int32_t* ffirst = nullptr;         // device: first descriptor of each subtree (+ end)
   int fsub = 0;                      // subtrees
+  // normalised evaluation of Float32 / ComplexF32 trees: the FP64 twin of this plan (same
+  // description, Float64 / ComplexF64 elements) and its widened, normalised inputs
+  std::vector<int32_t> desc;
+  ome_tree* wide = nullptr;
+  char* wleaves = nullptr;
+  std::vector<int64_t> wleaf_off;
 };
 
 namespace {
@@ -572,6 +578,7 @@ int ome_tree_create(ome_ctx* ctx, int dtype, const int32_t* desc, int64_t desc_l
     return tree_fail(ctx, Status::err(OME_ERR_INVALID_ARG, "malformed tree description"));
   }
   t->root_iy = t->steps.back().iy;
+  t->desc.assign(desc, desc + desc_len);
   // ---- slicing
   for (int q = 0; q < nslice; ++q) {
     int l = slice_labels[q];
@@ -772,24 +779,24 @@ int ome_tree_execute(ome_tree* t, const void* const* xs, void* y, const ome_scal
   return OME_OK;
 }
 
-int ome_tree_execute_scalar_normalized(ome_tree* t, const void* const* xs, double* value,
-                                       double* log_scale) {
-  if (!t || !xs || !value || !log_scale)
-    return tree_fail(t ? t->ctx : nullptr, Status::err(OME_ERR_INVALID_ARG, "null argument"));
+}  // extern "C"
+
+static double* norm_work(ome_tree* t) {
+  return (double*)(t->mem + t->arena_bytes + t->scratch_bytes + t->stage_bytes + t->acc_bytes);
+}
+
+// Every slice of t evaluated one step at a time, each intermediate divided by its largest
+// magnitude with the logs summed in FP64; the slice values (of t's element type) and logs
+// are combined on the host in FP64.  base_log is added to every slice's log.
+static Status normalized_slices(ome_tree* t, const std::vector<const void*>& xv, double base_log,
+                                double* value, double* log_scale) {
   ome_ctx* ctx = t->ctx;
-  if (prod(t->root_iy, t->sizes) != 1 || !is_numeric_float(t->dt))
-    return tree_fail(ctx, Status::err(OME_ERR_INVALID_ARG,
-                                      "normalised evaluation needs a float/complex scalar output"));
-  std::lock_guard<std::recursive_mutex> g(ctx->mu);
-  OptScope os(ctx, t->opt);
-  hipSetDevice(ctx->device);
   hipStream_t st = ctx->stream;
   {
     Status m = ensure_mem(t);
-    if (!m.good()) return tree_fail(ctx, m);
+    if (!m.good()) return m;
   }
-  std::vector<const void*> xv(xs, xs + t->n_leaves);
-  double* work = (double*)(t->mem + t->arena_bytes + t->scratch_bytes + t->stage_bytes + t->acc_bytes);
+  double* work = norm_work(t);
   double* logacc = work + 1 + 1024;
   const size_t es = elem_size(t->dt);
   Bump bump;
@@ -804,13 +811,13 @@ int ome_tree_execute_scalar_normalized(ome_tree* t, const void* const* xs, doubl
     stage_leaves(t, xv, s, leaves, st);
     Status r = run_steps(t, leaves, t->mem + t->slice_out_off, scal_one(t->dt), scal_zero(t->dt),
                          st, &bump, true, work, logacc);
-    if (!r.good()) return tree_fail(ctx, r);
+    if (!r.good()) return r;
     char host[16];
     double lg;
     if (hipMemcpyAsync(host, t->mem + t->slice_out_off, es, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync(&lg, logacc, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
-      return tree_fail(ctx, Status::err(OME_ERR_DEVICE, "copy of the scalar result failed"));
+      return Status::err(OME_ERR_DEVICE, "copy of the scalar result failed");
     double re = 0, im = 0;
     switch (t->dt) {
       case F32: re = *(float*)host; break;
@@ -818,11 +825,12 @@ int ome_tree_execute_scalar_normalized(ome_tree* t, const void* const* xs, doubl
       case C32: re = ((float*)host)[0]; im = ((float*)host)[1]; break;
       case C64: re = ((double*)host)[0]; im = ((double*)host)[1]; break;
     }
-    vr[s] = re; vi[s] = im; vl[s] = lg;
-    if (re != 0 || im != 0) lmax = std::max(lmax, lg);
+    vr[s] = re; vi[s] = im; vl[s] = lg + base_log;
+    if (re != 0 || im != 0) lmax = std::max(lmax, vl[s]);
   }
   if (!std::isfinite(lmax)) lmax = 0;
   for (int64_t s = 0; s < t->nslices; ++s) {
+    if (vr[s] == 0 && vi[s] == 0) continue;  // a zero slice adds nothing (and its log may be any)
     double f = std::exp(vl[s] - lmax);
     sum_re += vr[s] * f;
     sum_im += vi[s] * f;
@@ -830,7 +838,80 @@ int ome_tree_execute_scalar_normalized(ome_tree* t, const void* const* xs, doubl
   value[0] = sum_re;
   value[1] = sum_im;
   *log_scale = lmax;
-  return OME_OK;
+  return Status::ok();
+}
+
+// Float32 / ComplexF32 trees: every input tensor is divided by its largest magnitude while it
+// is widened to Float64 / ComplexF64 (the logs summed in FP64), and the plan's FP64 twin
+// evaluates the network with per-step normalisation, the final scalar included, in FP64.
+static Status normalized_wide(ome_tree* t, const void* const* xs, double* value, double* log_scale) {
+  ome_ctx* ctx = t->ctx;
+  hipStream_t st = ctx->stream;
+  const int wdt = t->dt == F32 ? F64 : C64;
+  if (!t->wide) {
+    const ome::Options saved = ctx->opt;
+    ctx->opt.defer_tree_mem = 0;
+    const int rc = ome_tree_create(ctx, wdt, t->desc.data(), (int64_t)t->desc.size(),
+                                   (int)t->sizes.size(), t->sizes.data(), (int)t->slicing.size(),
+                                   t->slicing.data(), &t->wide);
+    ctx->opt = saved;
+    if (rc != OME_OK) {
+      t->wide = nullptr;
+      return Status::err(rc, ctx->last_error);
+    }
+  }
+  ome_tree* w = t->wide;
+  {
+    Status m = ensure_mem(w);
+    if (!m.good()) return m;
+  }
+  const size_t wes = elem_size(wdt);
+  if (!t->wleaves) {
+    int64_t off = 0;
+    t->wleaf_off.assign(t->n_leaves, 0);
+    for (int k = 0; k < t->n_leaves; ++k) {
+      t->wleaf_off[k] = off;
+      off += align256(prod(t->leaf_ix[k], t->sizes) * (int64_t)wes);
+    }
+    t->wleaves = (char*)ctx->pool->alloc((size_t)std::max<int64_t>(off, 256));
+    if (!t->wleaves) return Status::err(OME_ERR_OOM, "device pool exhausted (widened inputs)");
+  }
+  double* work = norm_work(w);
+  double* leaflog = work + 1 + 1024 + 1;
+  launch_fill(F64, leaflog, 1, Scal{0, 0, 0}, st);
+  std::vector<const void*> wx(t->n_leaves);
+  for (int k = 0; k < t->n_leaves; ++k) {
+    const int64_t n = prod(t->leaf_ix[k], t->sizes);
+    wx[k] = t->wleaves + t->wleaf_off[k];
+    if (n > 0 && !xs[k]) return Status::err(OME_ERR_INVALID_ARG, "null input pointer");
+    launch_widen_normalized(t->dt, xs[k], t->wleaves + t->wleaf_off[k], n, work, leaflog, st);
+  }
+  double base = 0;
+  if (hipMemcpyAsync(&base, leaflog, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return Status::err(OME_ERR_DEVICE, "copy of the input scales failed");
+  OptScope os(ctx, w->opt);
+  return normalized_slices(w, wx, base, value, log_scale);
+}
+
+extern "C" {
+
+int ome_tree_execute_scalar_normalized(ome_tree* t, const void* const* xs, double* value,
+                                       double* log_scale) {
+  if (!t || !xs || !value || !log_scale)
+    return tree_fail(t ? t->ctx : nullptr, Status::err(OME_ERR_INVALID_ARG, "null argument"));
+  ome_ctx* ctx = t->ctx;
+  if (prod(t->root_iy, t->sizes) != 1 || !is_numeric_float(t->dt))
+    return tree_fail(ctx, Status::err(OME_ERR_INVALID_ARG,
+                                      "normalised evaluation needs a float/complex scalar output"));
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  OptScope os(ctx, t->opt);
+  hipSetDevice(ctx->device);
+  Status r = (t->dt == F32 || t->dt == C32)
+                 ? normalized_wide(t, xs, value, log_scale)
+                 : normalized_slices(t, std::vector<const void*>(xs, xs + t->n_leaves), 0.0, value,
+                                     log_scale);
+  return r.good() ? OME_OK : tree_fail(ctx, r);
 }
 
 int ome_tree_gradient(ome_tree* t, const void* const* xs, const void* ybar, void* y,
@@ -957,6 +1038,8 @@ int ome_tree_destroy(ome_tree* t) {
   if (t->mem && t->owns_mem) ctx->pool->free(t->mem);
   if (t->fdev) ctx->pool->free(t->fdev);
   if (t->ffirst) ctx->pool->free(t->ffirst);
+  if (t->wleaves) ctx->pool->free(t->wleaves);
+  if (t->wide) ome_tree_destroy(t->wide);
   delete t;
   return OME_OK;
 }

@@ -226,3 +226,20 @@ def test_c5_request_stream_alone_and_mixed(prec):
     print("C5 %s: error vs FP64 over 48 requests: median %.2e max %.2e (request %d)" %
           (prec, float(np.median(errs)), max(errs), int(np.argmax(errs))))
     assert max(errs) <= C5_BOUNDS[prec], (prec, errs)
+
+
+def test_c2_network_100_normalized_fp64_scalar():
+    # per-tensor and per-step log-scale normalisation, every step and the final scalar in
+    # FP64: the FP64 value of the committed ComplexF32 inputs, to FP64 rounding
+    o = om()
+    from omeinsum_mi355x.tree import plan_for
+    tree, sd, xs, ref = _network_100()
+    dxs = [torch.from_numpy(x).cuda() for x in xs]
+    o.set_einsum_backend(o.MI355XBackend(precision="fp32"))
+    try:
+        v, lg = plan_for(tree, dxs, sd).execute_scalar_normalized(dxs)
+    finally:
+        o.set_einsum_backend(o.DefaultBackend())
+    e = abs(v * np.exp(lg) - ref) / abs(ref)
+    print("C2 normalised FP64: error vs FP64 %.2e" % e)
+    assert e <= 1e-10, e

@@ -217,3 +217,54 @@ def test_grouped_permute_shared_dim_tiles(kind):
         torch.cuda.synchronize()
         for y, ref, p in zip(ys, refs, prev):
             assert np.array_equal(o.to_numpy(y), 2 * ref - p)
+
+
+def test_grouped_launches_replay_in_a_graph():
+    # a grouped GEMM and a grouped permute recorded once into a graph (after one eager run
+    # uploaded their descriptors) and replayed several times, with unrelated eager work and
+    # synchronisations in between: every replay equals the eager result bit for bit
+    o = om()
+    r = rng(3160)
+    ps = problems(3161, 48, np.float32, dims=range(8, 161, 8), batches=range(1, 5))
+    dev = [tuple(torch.from_numpy(np.ascontiguousarray(x)).cuda() for x in p) for p in ps]
+    xs, ys, perms = [], [], []
+    for _ in range(24):
+        shape = [int(s) for s in r.integers(4, 40, 4)]
+        xs.append(torch.from_numpy(r.uniform(-1, 1, shape).astype(np.float32)).cuda())
+        ys.append(torch.empty([shape[p] for p in (1, 3, 2, 0)], device="cuda"))
+        perms.append((1, 3, 2, 0))
+    gg = o.GroupedGemm(dev, 1.5, False)
+    gp = o.GroupedPermute(xs, perms, ys)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        gg()
+        gp()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    eager_c = [d[2].clone() for d in dev]
+    eager_y = [y.clone() for y in ys]
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        gg()
+        gp()
+    junk = torch.randn(512, 512, device="cuda")
+    for rep in range(4):
+        for d in dev:
+            d[2].fill_(float("nan"))
+        for y in ys:
+            y.fill_(float("nan"))
+        junk = junk @ junk.T / 512  # unrelated eager work between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        for d, c in zip(dev, eager_c):
+            assert torch.equal(d[2], c), rep
+        for y, e in zip(ys, eager_y):
+            assert torch.equal(y, e), rep
+    for (A, B, _), c in zip(ps, eager_c):
+        ref = 1.5 * np.matmul(A.astype(np.float64), B.astype(np.float64))
+        assert_close(c.cpu().numpy(), ref, np.float32, A.shape[2] + 1, depth=A.shape[2])
+    for x, y in zip(xs, eager_y):
+        assert torch.equal(y, x.permute(1, 3, 2, 0))
+    del graph

@@ -150,15 +150,24 @@ def test_normalized_scalar_and_sliced_fp64_accumulation():
     use("mi355x")
     from omeinsum_mi355x.tree import plan_for
     dxs = [dev(x) for x in xs]
+    # ComplexF32 inputs, evaluated in FP64 after per-tensor normalisation: the FP64 oracle's
+    # value of the same inputs to FP64 rounding
     v, lg = plan_for(tree, dxs, sd).execute_scalar_normalized(dxs)
     got = v * np.exp(lg)
-    assert abs(got - ref) <= 1e-4 * abs(ref), (got, ref)
+    assert abs(got - ref) <= 1e-12 * abs(ref), (got, ref)
     sliced = co.slice_code(tree, sd, sc_target=4)
     assert len(sliced.slicing) >= 1
     got_s = complex(om().to_numpy(sliced(*dxs))[()])
     assert abs(got_s - ref) <= 1e-4 * abs(ref), (got_s, ref)
     v2, lg2 = plan_for(sliced, dxs, sd).execute_scalar_normalized(dxs)
-    assert abs(v2 * np.exp(lg2) - ref) <= 1e-4 * abs(ref)
+    assert abs(v2 * np.exp(lg2) - ref) <= 1e-12 * abs(ref)
+    # inputs scaled by 2^100 (exactly, in ComplexF32): the value itself, 2^3000 times larger,
+    # is far outside Float64; the normalised result is the same value with a log larger by
+    # 30 * 100 ln 2
+    big = [dev((x * np.float32(2.0 ** 100)).astype(np.complex64)) for x in xs]
+    v3, lg3 = plan_for(tree, big, sd).execute_scalar_normalized(big)
+    assert abs(v3 * np.exp(lg3 - lg - len(xs) * 100 * np.log(2.0)) - v) <= 1e-12 * abs(v)
+    assert abs((lg3 - lg) - len(xs) * 100 * np.log(2.0)) <= 1e-9 * len(xs) * 100
 
 
 @pytest.mark.parametrize("backend", BACKENDS)
