@@ -10,8 +10,9 @@ from .codes import (ArgumentError, DimensionMismatch, EinCode, DynamicEinCode, S
                     NestedEinsum, DynamicNestedEinsum, StaticNestedEinsum, SlicedEinsum, ein,
                     parse_nested, get_size_dict, getixsv, getiyv, uniquelabels, labeltype,
                     flatten, uniformsize, drop_slicedim)
-from .contractionorder import (GreedyMethod, TreeSASlicer, optimize_code, slice_code, flop,
-                               peak_memory, contraction_complexity, writejson, readjson)
+from .contractionorder import (GreedyMethod, TreeSA, ScoreFunction, TreeSASlicer, optimize_code,
+                               slice_code, flop, peak_memory, contraction_complexity, writejson,
+                               readjson)
 
 _LAZY = {
     "einsum": "ein_ops", "einsum_inplace": "ein_ops", "loop_einsum": "ein_ops",

@@ -52,3 +52,24 @@ def _plain_from_json(fx, tmp="/tmp/ome_golden_tree.json"):
     with open(tmp, "w") as f:
         json.dump(fx["tree"], f)
     return co.readjson(tmp).to_plain()
+
+
+def test_request_stream_fixture_reproduces():
+    # the 48 requests come back from their seeds with the same graphs, greedy trees (flop
+    # counts) and complexities; the FP64 values of the six smallest are recomputed through the
+    # oracle (the whole stream, 5.2e12 flop, is left to `make_fixtures.py --stream`)
+    m = _load()
+    fx = json.load(open(os.path.join(HERE, "request_stream_48.json")))
+    stream = networks.request_stream(48, fx["seed"])
+    assert len(fx["requests"]) == 48
+    for q, fq in zip(stream, fx["requests"]):
+        assert (q["n"], q["graph_seed"], q["tensor_seed"]) == (fq["n"], fq["graph_seed"], fq["tensor_seed"])
+        code = networks.network_code(q["n"], q["edges"])
+        sd = {e: 2 for e in range(len(q["edges"]))}
+        tree = co.optimize_code(code, sd)
+        assert co.flop(tree, sd) == fq["flop"]
+        assert list(co.contraction_complexity(tree, sd))[:2] == fq["complexity"]
+    for i in sorted(range(48), key=lambda i: fx["requests"][i]["flop"])[:6]:
+        _, _, _, val = m.stream_request_value(stream[i])
+        ref = complex(*fx["requests"][i]["value_fp64"])
+        assert abs(val - ref) <= 1e-12 * abs(ref), i
