@@ -321,7 +321,7 @@ def bench_c2(steps, warmup):
     return out
 
 
-def bench_c3(steps, warmup):
+def bench_c3(steps, warmup, per_problem_path=True):
     """4096 uneven batched GEMMs (ein"ijb,jkb->ikb", i, j, k in {8, 16, ..., 256}, b in 1..16,
     Float32) in one grouped launch, plus 1024 ein"ijkl->ljik" permutes (dims in [4, 64]) in
     one grouped launch; against the per-problem path (one library call per GEMM / permute)."""
@@ -387,6 +387,8 @@ def bench_c3(steps, warmup):
             om.batched_gemm("N", "N", True, Bt, At, False, Ct)
         for x, y in zip(xs, ys):
             om.tensorpermute(y, x, (1, 3, 2, 0))
+    if not per_problem_path:  # (profiling runs of the grouped kernels leave it out)
+        return out
     out["per_problem_fp32_ms"] = time_events(per_problem, 2, 1)
     log("C3 per-problem path (one call per GEMM / permute): %.2f ms" % out["per_problem_fp32_ms"])
     # the CPU reference path on a sample (every 16th GEMM and permute; numpy / OpenBLAS),
@@ -653,12 +655,16 @@ def main():
                    default="all")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--copy-calibration", action="store_true")
+    p.add_argument("--no-calibration", action="store_true",
+                   help="do not run the MFMA-loop calibration program (a child process)")
+    p.add_argument("--no-per-problem", action="store_true",
+                   help="leave out the uneven-batch workload's one-call-per-problem timing")
     args = p.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this backend runs on a single MI355X (no multi-GPU "
               "path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    calib = mfma_calibration()
+    calib = None if args.no_calibration else mfma_calibration()
     import torch
     if not torch.cuda.is_available():
         print("bench.py needs the GPU", file=sys.stderr)
@@ -675,7 +681,7 @@ def main():
     if args.workload in ("all", "c2"):
         extra["c2_network_100"] = bench_c2(max(5, args.steps // 2), 2)
     if args.workload in ("all", "c3"):
-        extra["c3_grouped"] = bench_c3(max(3, args.steps // 4), 1)
+        extra["c3_grouped"] = bench_c3(max(3, args.steps // 4), 1, not args.no_per_problem)
     if args.workload in ("all", "c5"):
         extra["c5_serving"] = bench_c5()
     if args.copy_calibration:
