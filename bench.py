@@ -418,12 +418,14 @@ def bench_c3(steps, warmup, per_problem_path=True):
     return out
 
 
-def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
+def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8, request_limit=1 << 30):
     """Serving stream: requests are random 3-regular graphs with an even number of vertices in
     [40, 140], bond dimension 2, ComplexF32 Gaussian tensors, greedy order (the request list of
-    networks.request_stream, seed 5000); a request whose largest intermediate exceeds
-    2^sc_budget elements (2^25 ComplexF32 = 256 MiB, a plan of about 1 GiB) is sliced to fit
-    (FP64 cross-slice sum).
+    networks.request_stream, seed 5000).  The request loop takes the trees unsliced: a request
+    whose plan needs more than request_limit bytes (1 GiB) is sliced by the loop at
+    submission until it fits (FP64 cross-slice sum).  The per-step path takes the same trees
+    sliced on the host where the largest intermediate exceeds 2^sc_budget elements (2^25
+    ComplexF32 = 256 MiB).
     Inputs and plans are prepared before the timed runs (planning time reported apart).
     Measured: the plain per-step FP32 path back to back, the request loop at saturation (every
     request submitted at once) in FP32 and in BF16x3, and each loop under Poisson arrivals at
@@ -447,15 +449,17 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
         tree = co.optimize_code(code, sd)
         if co.flop(tree, sd) != fq["flop"]:
             raise RuntimeError("request tree differs from the fixture's")
+        host_sliced = tree
         if co.contraction_complexity(tree, sd)[1] > sc_budget:
-            tree = co.slice_code(tree, sd, sc_target=sc_budget)
+            host_sliced = co.slice_code(tree, sd, sc_target=sc_budget)
         f = 8.0 * co.flop(tree, sd)
         flops += f
-        reqs.append((spec["n"], tree, sd, [torch.from_numpy(x).cuda() for x in xs], f))
+        reqs.append((spec["n"], tree, sd, [torch.from_numpy(x).cuda() for x in xs], f, host_sliced))
         refs.append(complex(*fq["value_fp64"]))
     t_plan = time.perf_counter() - t_plan
-    sliced = sum(1 for q in reqs if hasattr(q[1], "slicing") and q[1].slicing)
-    out = {"requests": nreq, "sliced": sliced, "sc_budget_log2_elems": sc_budget,
+    sliced = sum(1 for q in reqs if hasattr(q[5], "slicing") and q[5].slicing)
+    out = {"requests": nreq, "host_sliced_for_per_step_path": sliced,
+           "sc_budget_log2_elems": sc_budget, "request_limit_bytes": request_limit,
            "total_flop": flops, "host_planning_s": t_plan}
 
     def scalar(y):
@@ -468,14 +472,14 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
 
     # 1. the plain per-step path (one einsum per tree node from the host, new output per node)
     om.set_einsum_backend(om.DefaultBackend())
-    om.einsum(reqs[0][1], reqs[0][3], reqs[0][2])
+    om.einsum(reqs[0][5], reqs[0][3], reqs[0][2])
     torch.cuda.synchronize()
     per_step = []
     lat = []
     t0 = time.perf_counter()
-    for n, tree, sd, dxs, f in reqs:
+    for n, tree, sd, dxs, f, hs in reqs:
         a = time.perf_counter()
-        per_step.append(scalar(om.einsum(tree, dxs, sd)))
+        per_step.append(scalar(om.einsum(hs, dxs, sd)))
         lat.append((time.perf_counter() - a) * 1e3)
     t_ps = time.perf_counter() - t0
     out["per_step"] = {"req_per_s": nreq / t_ps, "p50_ms": float(np.percentile(lat, 50)),
@@ -487,9 +491,10 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
 
     def run_loop(backend):
         res_out = {}
-        with Server(hbm_budget=hbm_budget, max_active=max_active, backend=backend) as srv:
+        with Server(hbm_budget=hbm_budget, max_active=max_active, backend=backend,
+                    request_limit=request_limit) as srv:
             t0 = time.perf_counter()
-            plans = [srv.plan(tree, dxs, sd) for n, tree, sd, dxs, f in reqs]
+            plans = [srv.plan(tree, dxs, sd) for n, tree, sd, dxs, f, hs in reqs]
             res_out["device_planning_s"] = time.perf_counter() - t0
             # warm-up pass
             for t in [srv.submit(p, q[3]) for p, q in zip(plans, reqs)]:
@@ -522,6 +527,7 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
             res_out["poisson"] = {"offered_req_per_s": rate, "req_per_s": nreq / t_poi,
                                   "p50_ms": float(np.percentile(lats, 50)),
                                   "p99_ms": float(np.percentile(lats, 99))}
+            res_out["loop_stats"] = srv.stats()
         res_out["speedup_req_per_s"] = sat_rps / out["per_step"]["req_per_s"]
         res_out["rel_err_fp64"] = err_stats(vals)
         res_out["max_rel_diff_vs_per_step"] = max(abs(a - b) / abs(b) for a, b in zip(vals, per_step))
@@ -541,9 +547,9 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8):
     # the CPU reference path on the small requests (oracle per-step restatement, FP64), timed
     from oracle import default_backend as db
     t_cpu, n_cpu = 0.0, 0
-    for q, (n, tree, sd, dxs, f) in enumerate(reqs):
+    for q, (n, tree, sd, dxs, f, hs) in enumerate(reqs):
         if f <= 2e9:
-            plain = tree.eins if hasattr(tree, "slicing") else tree
+            plain = tree
             xs64 = [x.cpu().numpy().astype(np.complex128) for x in dxs]
             t1 = time.perf_counter()
             db.nested_einsum(plain.to_plain(), xs64, sd)

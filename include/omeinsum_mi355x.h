@@ -319,6 +319,20 @@ int ome_server_poll(ome_server* srv, int64_t ticket, int* status);
 /* Blocks until done; latency_ms = completion - submission (host wall clock). */
 int ome_server_wait(ome_server* srv, int64_t ticket, double* latency_ms);
 int ome_server_destroy(ome_server* srv);
+/* Per-request memory limit (default: the whole budget).  A request whose plan needs more is
+ * sliced at submission (src/slicing.jl:104-137 semantics): the loop admits a twin of its tree
+ * with labels sliced one at a time (each time the non-output label carried by the most
+ * elements of the step outputs) until the twin's memory fits, built once per plan and limit.
+ * Submission fails with OME_ERR_OOM when no slicing fits. */
+int ome_server_set_request_limit(ome_server* srv, size_t bytes);
+typedef struct ome_server_stats {
+  int64_t submitted;
+  int64_t completed;
+  int64_t sliced;         /* submissions admitted as a sliced twin */
+  int64_t blocks_reused;  /* admissions lent the block the same plan used last */
+  int64_t kept_bytes;     /* retired blocks the loop keeps for later admissions */
+} ome_server_stats;
+int ome_server_get_stats(ome_server* srv, ome_server_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -158,5 +158,7 @@ Scal scal_zero(int dt);
 int64_t tree_mem_bytes(const ome_tree* t);
 Status tree_run_lent(ome_tree* t, const void* const* xs, void* y, char* mem, hipStream_t st);
 void tree_return_lent(ome_tree* t);
+// The plan to admit under a per-request memory limit: t, or a sliced twin that fits.
+Status tree_fit(ome_tree* t, int64_t cap, ome_tree** out);
 
 }  // namespace ome

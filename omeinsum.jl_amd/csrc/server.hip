@@ -7,14 +7,21 @@
 //      frees the slot;
 //   2. admit: queued requests are scanned in arrival order and every one whose plan memory
 //      (arena + scratch + staging + accumulators, known exactly at plan time) fits the
-//      remaining HBM budget is started on a free slot: memory from the pool is lent to the
-//      plan, the slot's stream waits for the context's stream as it stands at admission (the
-//      inputs are ready, and so is every earlier use of the lent memory: the pool hands out
-//      blocks freed on the host as soon as work on them was enqueued on the context's
-//      stream), the whole evaluation is enqueued, and a completion event is recorded;
+//      remaining HBM budget is started on a free slot: a memory block is lent to the plan
+//      (the block this plan used last when it is free, else the smallest free block that
+//      fits, else a new one from the pool), the slot's stream waits for the context's stream
+//      as it stands at admission (the inputs are ready, and so is every earlier use of the
+//      lent memory: the pool hands out blocks freed on the host as soon as work on them was
+//      enqueued on the context's stream), the evaluation is launched (the plan's recorded
+//      graph when it was lent this block before, so the steps are not planned again), and a
+//      completion event is recorded;
 //   3. sleep briefly while requests run, or block until the next submission when idle.
 // Each slot has its own stream, so the steps of admitted requests overlap on the device, and
 // requests that arrive while others run are admitted as soon as a slot and memory are free.
+// Retired blocks stay with the loop (within the budget) for the next admissions.  A request
+// whose plan needs more than the per-request limit (the whole budget unless set) is sliced at
+// submission: the loop admits a twin of its tree with more labels sliced, whose memory fits
+// (SlicedEinsum semantics, FP64 cross-slice sum).
 // Sliced requests (SlicedEinsum plans) keep their FP64 cross-slice accumulator in the lent
 // memory.  A tree must not be used elsewhere while its request is queued or running, and the
 // context's stream must not be under graph capture while the loop has queued requests.
@@ -44,8 +51,15 @@ struct Req {
   double t_submit = 0, t_start = 0, t_done = 0;
   int status = 0;  // 0 queued, 1 running, 2 done, < 0 failed (negated ome_status)
   std::string msg;
-  size_t bytes = 0;
+  size_t bytes = 0;     // plan memory
+  size_t held = 0;      // bytes of the lent block (>= bytes)
   char* mem = nullptr;
+};
+
+struct Block {
+  char* p = nullptr;
+  size_t bytes = 0;
+  ome_tree* last = nullptr;  // the plan that used it last
 };
 
 struct Slot {
@@ -60,6 +74,10 @@ struct Slot {
 struct ome_server {
   ome_ctx* ctx = nullptr;
   size_t budget = 0, in_use = 0;
+  size_t req_limit = 0;            // per-request plan memory; larger requests are sliced
+  std::vector<Block> free_blocks;  // retired blocks kept for the next admissions
+  size_t free_bytes = 0;
+  ome_server_stats stats{};
   std::vector<Slot> slots;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -76,9 +94,17 @@ namespace {
 void retire(ome_server* s, Slot& sl, int status, const std::string& msg) {
   Req* r = sl.req;
   tree_return_lent(r->tree);
-  if (r->mem) s->ctx->pool->free(r->mem);
+  if (r->mem) {
+    if (status == 2) {  // kept for the next admissions, tagged with its last plan
+      s->free_blocks.push_back(Block{r->mem, r->held, r->tree});
+      s->free_bytes += r->held;
+    } else {
+      s->ctx->pool->free(r->mem);
+    }
+  }
   r->mem = nullptr;
-  s->in_use -= r->bytes;
+  s->in_use -= r->held;
+  if (status == 2) ++s->stats.completed;
   r->t_done = now_ms();
   r->status = status;
   r->msg = msg;
@@ -114,7 +140,45 @@ void scheduler(ome_server* s) {
       Req* r = *it;
       if (r->bytes > s->budget - s->in_use) { ++it; continue; }
       it = s->queue.erase(it);
-      char* mem = r->bytes ? (char*)s->ctx->pool->alloc(r->bytes) : nullptr;
+      char* mem = nullptr;
+      size_t held = r->bytes;
+      if (r->bytes) {
+        // the block this plan used last; else a new block while the budget has room beside
+        // the kept ones (so that other plans find theirs); else the smallest kept block
+        // that fits
+        int pick = -1;
+        for (size_t b = 0; b < s->free_blocks.size(); ++b) {
+          const Block& bl = s->free_blocks[b];
+          if (bl.last == r->tree && bl.bytes >= r->bytes && bl.bytes <= s->budget - s->in_use) {
+            pick = (int)b;
+            break;
+          }
+        }
+        if (pick < 0 && s->in_use + s->free_bytes + r->bytes > s->budget) {
+          for (size_t b = 0; b < s->free_blocks.size(); ++b) {
+            const Block& bl = s->free_blocks[b];
+            if (bl.bytes < r->bytes || bl.bytes > s->budget - s->in_use || bl.bytes > 2 * r->bytes)
+              continue;
+            if (pick < 0 || bl.bytes < s->free_blocks[pick].bytes) pick = (int)b;
+          }
+        }
+        if (pick >= 0) {
+          mem = s->free_blocks[pick].p;
+          held = s->free_blocks[pick].bytes;
+          if (s->free_blocks[pick].last == r->tree) ++s->stats.blocks_reused;
+          s->free_bytes -= held;
+          s->free_blocks.erase(s->free_blocks.begin() + pick);
+        } else {
+          // a new block: kept blocks go back to the pool (oldest first) while the kept and
+          // the running memory together would exceed the budget
+          while (!s->free_blocks.empty() && s->in_use + s->free_bytes + r->bytes > s->budget) {
+            s->ctx->pool->free(s->free_blocks.front().p);
+            s->free_bytes -= s->free_blocks.front().bytes;
+            s->free_blocks.erase(s->free_blocks.begin());
+          }
+          mem = (char*)s->ctx->pool->alloc(r->bytes);
+        }
+      }
       if (r->bytes && !mem) {
         // the pool cannot serve the budget it was promised: fail this request
         r->status = -OME_ERR_OOM;
@@ -124,9 +188,10 @@ void scheduler(ome_server* s) {
         continue;
       }
       r->mem = mem;
+      r->held = held;
       r->status = 1;
       r->t_start = now_ms();
-      s->in_use += r->bytes;
+      s->in_use += held;
       ++s->active;
       free_slot->req = r;
       Slot& sl = *free_slot;
@@ -189,6 +254,7 @@ int ome_server_create(ome_ctx* ctx, size_t hbm_budget, int max_active, ome_serve
   ome_server* s = new ome_server();
   s->ctx = ctx;
   s->budget = hbm_budget;
+  s->req_limit = hbm_budget;
   s->slots.resize(max_active);
   for (auto& sl : s->slots) {
     if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
@@ -214,16 +280,24 @@ int ome_server_submit(ome_server* s, ome_tree* tree, const void* const* xs, void
   if (!s || !tree || !ticket) return sfail(s, OME_ERR_INVALID_ARG, "null argument");
   ome_tree_info info{};
   ome_get_tree_info(tree, &info);
+  size_t limit;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    limit = std::min(s->req_limit, s->budget);
+  }
+  // a plan larger than the per-request limit is sliced here (once per plan and limit)
+  ome_tree* fit = nullptr;
+  Status fs = tree_fit(tree, (int64_t)limit, &fit);
+  if (!fs.good())
+    return sfail(s, fs.code == OME_ERR_OOM ? OME_ERR_OOM : fs.code,
+                 "request needs " + std::to_string(tree_mem_bytes(tree)) +
+                     " bytes of device memory, above the per-request limit of " +
+                     std::to_string(limit) + ", and slicing does not fit it: " + fs.msg);
   Req* r = new Req();
-  r->tree = tree;
+  r->tree = fit;
   r->xs.assign(xs, xs + info.n_leaves);
   r->y = y;
-  r->bytes = (size_t)tree_mem_bytes(tree);
-  if (r->bytes > s->budget) {
-    delete r;
-    return sfail(s, OME_ERR_OOM, "request needs " + std::to_string(tree_mem_bytes(tree)) +
-                                     " bytes of device memory, more than the request-loop budget");
-  }
+  r->bytes = (size_t)tree_mem_bytes(fit);
   hipSetDevice(s->ctx->device);
   hipStream_t submit_stream;
   {
@@ -241,6 +315,8 @@ int ome_server_submit(ome_server* s, ome_tree* tree, const void* const* xs, void
     std::lock_guard<std::mutex> lk(s->mu);
     r->ticket = s->next++;
     r->t_submit = now_ms();
+    ++s->stats.submitted;
+    if (fit != tree) ++s->stats.sliced;
     s->reqs[r->ticket] = r;
     s->queue.push_back(r);
   }
@@ -295,7 +371,23 @@ int ome_server_destroy(ome_server* s) {
     hipEventDestroy(kv.second->ready);
     delete kv.second;
   }
+  for (auto& b : s->free_blocks) s->ctx->pool->free(b.p);
   delete s;
+  return OME_OK;
+}
+
+int ome_server_set_request_limit(ome_server* s, size_t bytes) {
+  if (!s || bytes == 0) return sfail(s, OME_ERR_INVALID_ARG, "invalid request limit");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->req_limit = bytes;
+  return OME_OK;
+}
+
+int ome_server_get_stats(ome_server* s, ome_server_stats* stats) {
+  if (!s || !stats) return sfail(s, OME_ERR_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *stats = s->stats;
+  stats->kept_bytes = (int64_t)s->free_bytes;
   return OME_OK;
 }
 

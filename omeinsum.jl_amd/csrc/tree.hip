@@ -98,6 +98,7 @@ struct TreeStep {
 };
 
 struct GraphEntry {
+  char* mem;  // the plan's memory when the graph was recorded (its own or lent)
   std::vector<const void*> xs;
   void* y;
   Scal sx, sy;
@@ -141,6 +142,9 @@ struct ome_tree {
   ome_tree* wide = nullptr;
   char* wleaves = nullptr;
   std::vector<int64_t> wleaf_off;
+  // the request loop's admission: a sliced twin whose memory fits fit_cap bytes
+  ome_tree* fit = nullptr;
+  int64_t fit_cap = 0;
 };
 
 namespace {
@@ -459,7 +463,8 @@ static void accumulate_slice(ome_tree* t, int64_t sidx, void* acc, hipStream_t s
 }
 
 // Give the plan its own device memory if it has none (plans created with deferred memory,
-// or whose memory a request loop took back).  Recorded graphs hold the old addresses.
+// or whose memory a request loop took back).  Recorded graphs are keyed by the memory they
+// were recorded with and replay only while the plan holds that memory again.
 static Status ensure_mem(ome_tree* t) {
   if (t->mem) return Status::ok();
   t->mem = (char*)t->ctx->pool->alloc((size_t)t->mem_bytes);
@@ -467,8 +472,6 @@ static Status ensure_mem(ome_tree* t) {
     return Status::err(OME_ERR_OOM, "device pool exhausted (tree arena of " +
                                         std::to_string(t->mem_bytes) + " bytes)");
   t->owns_mem = true;
-  for (auto& ge : t->graphs) hipGraphExecDestroy(ge.exec);
-  t->graphs.clear();
   return Status::ok();
 }
 
@@ -498,6 +501,60 @@ static Status enqueue_eval(ome_tree* t, const std::vector<const void*>& xs, void
   }
   if (t->acc_f64) launch_from_f64(t->dt, (const double*)acc, y, ny, sx, sy, st);
   else launch_axpby(t->dt, acc, y, ny, sx, sy, 0, st);
+  return Status::ok();
+}
+
+// One full evaluation as a graph launched on `st`: the graph recorded for the plan's current
+// memory, inputs, output and scalars is replayed (LRU of 8 per plan), or recorded first on
+// the plan's capture stream.  Falls back to direct launches when capture is unavailable.
+static Status launch_graphed(ome_tree* t, const std::vector<const void*>& xv, void* y,
+                             const Scal& sx, const Scal& sy, hipStream_t st) {
+  auto same = [&](const GraphEntry& e) {
+    return e.mem == t->mem && e.xs == xv && e.y == y && std::memcmp(&e.sx, &sx, sizeof sx) == 0 &&
+           std::memcmp(&e.sy, &sy, sizeof sy) == 0;
+  };
+  auto it = std::find_if(t->graphs.begin(), t->graphs.end(), same);
+  if (it == t->graphs.end()) {
+    hipGraph_t graph;
+    if (hipStreamBeginCapture(t->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      hipGetLastError();
+      return enqueue_eval(t, xv, y, sx, sy, st);
+    }
+    Status r = enqueue_eval(t, xv, y, sx, sy, t->cap_stream);
+    hipError_t e = hipStreamEndCapture(t->cap_stream, &graph);
+    if (!r.good()) {
+      if (e == hipSuccess) hipGraphDestroy(graph);
+      return r;
+    }
+    if (e != hipSuccess) {
+      hipGetLastError();
+      return enqueue_eval(t, xv, y, sx, sy, st);
+    }
+    GraphEntry ge;
+    ge.mem = t->mem;
+    ge.xs = xv;
+    ge.y = y;
+    ge.sx = sx;
+    ge.sy = sy;
+    e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      hipGetLastError();
+      return enqueue_eval(t, xv, y, sx, sy, st);
+    }
+    t->graphs.push_front(ge);
+    if (t->graphs.size() > 8) {
+      hipGraphExecDestroy(t->graphs.back().exec);
+      t->graphs.pop_back();
+    }
+    it = t->graphs.begin();
+  } else if (it != t->graphs.begin()) {
+    t->graphs.splice(t->graphs.begin(), t->graphs, it);
+    it = t->graphs.begin();
+  }
+  const hipError_t e = hipGraphLaunch(it->exec, st);
+  if (e != hipSuccess)
+    return Status::err(OME_ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e));
   return Status::ok();
 }
 
@@ -730,53 +787,7 @@ int ome_tree_execute(ome_tree* t, const void* const* xs, void* y, const ome_scal
   }
   if (!ctx->opt.use_graphs || !t->cap_stream)
     return tree_fail(ctx, enqueue_eval(t, xv, y, sx, sy, ctx->stream)) ;
-  // graph cache keyed by pointers and scalars
-  auto same = [&](const GraphEntry& e) {
-    return e.xs == xv && e.y == y && std::memcmp(&e.sx, &sx, sizeof sx) == 0 &&
-           std::memcmp(&e.sy, &sy, sizeof sy) == 0;
-  };
-  auto it = std::find_if(t->graphs.begin(), t->graphs.end(), same);
-  if (it == t->graphs.end()) {
-    hipGraph_t graph;
-    if (hipStreamBeginCapture(t->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-      hipGetLastError();
-      return tree_fail(ctx, enqueue_eval(t, xv, y, sx, sy, ctx->stream));
-    }
-    Status r = enqueue_eval(t, xv, y, sx, sy, t->cap_stream);
-    hipError_t e = hipStreamEndCapture(t->cap_stream, &graph);
-    if (!r.good()) {
-      if (e == hipSuccess) hipGraphDestroy(graph);
-      return tree_fail(ctx, r);
-    }
-    if (e != hipSuccess) {
-      hipGetLastError();
-      return tree_fail(ctx, enqueue_eval(t, xv, y, sx, sy, ctx->stream));
-    }
-    GraphEntry ge;
-    ge.xs = xv;
-    ge.y = y;
-    ge.sx = sx;
-    ge.sy = sy;
-    e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
-    if (e != hipSuccess) {
-      hipGetLastError();
-      return tree_fail(ctx, enqueue_eval(t, xv, y, sx, sy, ctx->stream));
-    }
-    t->graphs.push_front(ge);
-    if (t->graphs.size() > 8) {
-      hipGraphExecDestroy(t->graphs.back().exec);
-      t->graphs.pop_back();
-    }
-    it = t->graphs.begin();
-  } else if (it != t->graphs.begin()) {
-    t->graphs.splice(t->graphs.begin(), t->graphs, it);
-    it = t->graphs.begin();
-  }
-  hipError_t e = hipGraphLaunch(it->exec, ctx->stream);
-  if (e != hipSuccess)
-    return tree_fail(ctx, Status::err(OME_ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e)));
-  return OME_OK;
+  return tree_fail(ctx, launch_graphed(t, xv, y, sx, sy, ctx->stream));
 }
 
 }  // extern "C"
@@ -1040,6 +1051,7 @@ int ome_tree_destroy(ome_tree* t) {
   if (t->ffirst) ctx->pool->free(t->ffirst);
   if (t->wleaves) ctx->pool->free(t->wleaves);
   if (t->wide) ome_tree_destroy(t->wide);
+  if (t->fit) ome_tree_destroy(t->fit);
   delete t;
   return OME_OK;
 }
@@ -1061,8 +1073,6 @@ Status tree_run_lent(ome_tree* t, const void* const* xs, void* y, char* mem, hip
   hipSetDevice(ctx->device);
   if (t->mem != mem) {
     if (t->mem && t->owns_mem) ctx->pool->free(t->mem);
-    for (auto& ge : t->graphs) hipGraphExecDestroy(ge.exec);
-    t->graphs.clear();
     t->mem = mem;
     t->owns_mem = false;
   }
@@ -1071,7 +1081,71 @@ Status tree_run_lent(ome_tree* t, const void* const* xs, void* y, char* mem, hip
   for (int k = 0; k < t->n_leaves; ++k)
     if (!xv[k] && prod(t->leaf_ix[k], t->sizes) != 0)
       return Status::err(OME_ERR_INVALID_ARG, "null input pointer");
+  // a plan lent the same memory again (the loop hands blocks back to the plan that used them)
+  // replays its recorded graph: no host planning of the steps
+  if (t->opt.use_graphs && t->cap_stream)
+    return launch_graphed(t, xv, y, scal_one(t->dt), scal_zero(t->dt), st);
   return enqueue_eval(t, xv, y, scal_one(t->dt), scal_zero(t->dt), st);
+}
+
+// The plan the request loop admits for t under a per-request limit of cap bytes: t itself
+// when its memory fits; else a twin of the same tree with more labels sliced (kept with t and
+// reused while the limit is unchanged).  Labels are added one at a time, each time the label
+// (not in the output, of size > 1) carried by the most elements of the step outputs, until
+// the twin's memory fits; fails when slicing every such label does not make it fit.
+Status tree_fit(ome_tree* t, int64_t cap, ome_tree** out) {
+  *out = t;
+  if (t->mem_bytes <= cap) return Status::ok();
+  ome_ctx* ctx = t->ctx;
+  std::lock_guard<std::recursive_mutex> g(ctx->mu);
+  if (t->fit && t->fit_cap == cap) {
+    *out = t->fit;
+    return Status::ok();
+  }
+  if (t->fit) {
+    ome_tree_destroy(t->fit);
+    t->fit = nullptr;
+  }
+  std::vector<int> slicing = t->slicing;
+  std::vector<int64_t> sz = t->sizes;
+  for (int l : slicing) sz[l] = 1;
+  OptScope os(ctx, t->opt);
+  const ome::Options saved = ctx->opt;
+  ctx->opt.defer_tree_mem = 1;
+  Status result = Status::err(OME_ERR_OOM, "no slicing of this tree fits " + std::to_string(cap) +
+                                               " bytes of device memory");
+  for (;;) {
+    std::vector<double> score(sz.size(), 0.0);
+    for (const auto& st : t->steps) {
+      const double e = (double)prod(st.iy, sz);
+      for (int l : st.iy)
+        if (sz[l] > 1 && !has(t->root_iy, l)) score[l] += e;
+    }
+    int best = -1;
+    for (size_t l = 0; l < score.size(); ++l)
+      if (score[l] > 0 && (best < 0 || score[l] > score[best])) best = (int)l;
+    if (best < 0) break;
+    slicing.push_back(best);
+    sz[best] = 1;
+    ome_tree* c = nullptr;
+    const int rc = ome_tree_create(ctx, t->dt, t->desc.data(), (int64_t)t->desc.size(),
+                                   (int)t->sizes.size(), t->sizes.data(), (int)slicing.size(),
+                                   slicing.data(), &c);
+    if (rc != OME_OK) {
+      result = Status::err(rc, ctx->last_error);
+      break;
+    }
+    if (c->mem_bytes <= cap) {
+      t->fit = c;
+      t->fit_cap = cap;
+      *out = c;
+      result = Status::ok();
+      break;
+    }
+    ome_tree_destroy(c);
+  }
+  ctx->opt = saved;
+  return result;
 }
 
 // Forget lent memory (the lender frees it).

@@ -44,6 +44,11 @@ class PoolStats(ctypes.Structure):
                 ("peak_in_use_bytes", c_i64), ("n_allocs", c_i64), ("n_device_mallocs", c_i64)]
 
 
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", c_i64), ("completed", c_i64), ("sliced", c_i64),
+                ("blocks_reused", c_i64), ("kept_bytes", c_i64)]
+
+
 class TreeInfo(ctypes.Structure):
     _fields_ = [("n_steps", c_i32), ("n_leaves", c_i32), ("n_slices", c_i64),
                 ("flops", c_double), ("bytes", c_double), ("arena_bytes", c_i64),
@@ -99,6 +104,8 @@ SIGNATURES = {
     "ome_server_poll": (c_int, [c_void_p, c_i64, P(c_int)]),
     "ome_server_wait": (c_int, [c_void_p, c_i64, P(c_double)]),
     "ome_server_destroy": (c_int, [c_void_p]),
+    "ome_server_set_request_limit": (c_int, [c_void_p, c_size_t]),
+    "ome_server_get_stats": (c_int, [c_void_p, P(ServerStats)]),
 }
 
 _lib = None

@@ -20,7 +20,8 @@ from .tree import TreePlan, element_code
 
 
 class Server:
-    def __init__(self, hbm_budget=8 << 30, max_active=8, backend=None, device=None):
+    def __init__(self, hbm_budget=8 << 30, max_active=8, backend=None, device=None,
+                 request_limit=None):
         import torch
         self.ctx = _lib.context(device)
         self.backend = backend if backend is not None else MI355XBackend()
@@ -29,6 +30,9 @@ class Server:
                                                   ctypes.byref(h)), self.ctx.h)
         self.h = h
         self.budget = int(hbm_budget)
+        if request_limit is not None:
+            _lib.check(self.ctx.lib.ome_server_set_request_limit(self.h, int(request_limit)),
+                       self.ctx.h)
         self._live = {}
         self._torch = torch
 
@@ -70,6 +74,13 @@ class Server:
         finally:
             entry = self._live.pop(ticket, None)
         return entry[2], lat.value
+
+    def stats(self):
+        """Counters of the loop: submitted, completed, sliced (admitted as a sliced twin),
+        blocks_reused (lent the block the same plan used last), kept_bytes."""
+        st = _lib.ServerStats()
+        _lib.check(self.ctx.lib.ome_server_get_stats(self.h, ctypes.byref(st)), self.ctx.h)
+        return {k: getattr(st, k) for k, _ in st._fields_}
 
     def close(self):
         if getattr(self, "h", None):

@@ -177,7 +177,7 @@ def test_c2_network_100_scalar(kind):
     assert e <= C2_BOUNDS[kind], (kind, e)
 
 
-def _stream(sc_budget=25):
+def _stream(sc_budget=None):
     from omeinsum_mi355x import contractionorder as co, networks
     fx = json.load(open(os.path.join(GOLDEN, "request_stream_48.json")))
     reqs = []
@@ -187,7 +187,7 @@ def _stream(sc_budget=25):
         sd = {e: 2 for e in range(len(spec["edges"]))}
         tree = co.optimize_code(code, sd)
         assert co.flop(tree, sd) == fq["flop"], q   # the same tree as the fixture's
-        if co.contraction_complexity(tree, sd)[1] > sc_budget:
+        if sc_budget is not None and co.contraction_complexity(tree, sd)[1] > sc_budget:
             tree = co.slice_code(tree, sd, sc_target=sc_budget)
         xs = networks.gaussian_tensors(code, sd, spec["tensor_seed"], np.complex64)
         reqs.append((tree, sd, [torch.from_numpy(x).cuda() for x in xs],
@@ -207,18 +207,22 @@ C5_BOUNDS = {"fp32": 1e-5, "bf16x3": 3e-4}
 def test_c5_request_stream_alone_and_mixed(prec):
     o = om()
     from omeinsum_mi355x.server import Server
+    # the requests go in unsliced: those whose plans need more than the 1 GiB per-request
+    # limit are sliced by the loop at submission
     reqs = _stream()
-    assert sum(1 for r in reqs if getattr(r[0], "slicing", None)) == 4
     be = o.MI355XBackend(precision=prec)
-    with Server(hbm_budget=16 << 30, max_active=1, backend=be) as srv:
+    kw = dict(hbm_budget=16 << 30, backend=be, request_limit=1 << 30)
+    with Server(max_active=1, **kw) as srv:
         alone = []
         for tree, sd, dxs, _ in reqs:
             y, _ = srv.wait(srv.submit(srv.plan(tree, dxs, sd), dxs))
             alone.append(complex(o.to_numpy(y)[()]))
-    with Server(hbm_budget=16 << 30, max_active=8, backend=be) as srv:
+        assert srv.stats()["sliced"] >= 4
+    with Server(max_active=8, **kw) as srv:
         plans = [srv.plan(tree, dxs, sd) for tree, sd, dxs, _ in reqs]
         tickets = [srv.submit(p, r[2]) for p, r in zip(plans, reqs)]
         mixed = [complex(o.to_numpy(srv.wait(t)[0])[()]) for t in tickets]
+        print("C5 %s loop stats: %s" % (prec, srv.stats()))
     errs = []
     for q, ((_, _, _, ref), a, m) in enumerate(zip(reqs, alone, mixed)):
         assert abs(m - a) <= 1e-6 * abs(a), (q, m, a)

@@ -92,12 +92,43 @@ def test_budget_admission_and_rejection():
         tickets = [srv.submit(p, dxs) for p in plans]
         outs = [complex(om().to_numpy(srv.wait(t)[0])[()]) for t in tickets]
         assert len(set(outs)) == 1
-    # a plan larger than the whole budget is refused at submission
+    # a plan larger than the whole budget is sliced at submission until it fits; a budget
+    # no slicing fits refuses the request
+    ref = fp64(tree, sd, xs)
     with Server(hbm_budget=max(need // 2, 1), max_active=2) as srv:
+        plan = srv.plan(tree, dxs, sd)
+        got = complex(om().to_numpy(srv.wait(srv.submit(plan, dxs))[0])[()])
+        assert srv.stats()["sliced"] == 1
+    assert abs(got - ref) <= 1e-5 * abs(ref), (got, ref)
+    with Server(hbm_budget=4096, max_active=2) as srv:
         plan = srv.plan(tree, dxs, sd)
         with pytest.raises(MemoryError):
             srv.submit(plan, dxs)
     assert isinstance(_lib.OPT_TREE_DEFER_MEMORY, int)
+
+
+def test_request_limit_slices_at_submission_and_blocks_return_to_their_plan():
+    # a per-request limit below a plan's memory: the loop admits a sliced twin (FP64
+    # cross-slice sum) whose memory fits; resubmitting a plan lends it the block it used
+    # before, and its recorded graph replays with the same result
+    from omeinsum_mi355x.server import Server, plan_bytes
+    reqs = [request(n, 5500 + i) for i, n in enumerate([56, 64, 44, 60])]
+    with Server(hbm_budget=1 << 30, max_active=2) as probe:
+        sizes = [plan_bytes(probe.plan(tree, dxs, sd)) for tree, sd, xs, dxs in reqs]
+    limit = sorted(sizes)[1]  # the two largest plans exceed it
+    with Server(hbm_budget=1 << 30, max_active=2, request_limit=limit) as srv:
+        plans = [srv.plan(tree, dxs, sd) for tree, sd, xs, dxs in reqs]
+        first = [complex(om().to_numpy(srv.wait(srv.submit(p, r[3]))[0])[()])
+                 for p, r in zip(plans, reqs)]
+        st = srv.stats()
+        assert st["sliced"] == sum(1 for b in sizes if b > limit) >= 1
+        again = [complex(om().to_numpy(srv.wait(srv.submit(p, r[3]))[0])[()])
+                 for p, r in zip(plans, reqs)]
+        assert srv.stats()["blocks_reused"] >= 1
+    for (tree, sd, xs, _), a, b in zip(reqs, first, again):
+        ref = fp64(tree, sd, xs)
+        assert abs(a - ref) <= 1e-5 * abs(ref), (a, ref)
+        assert a == b
 
 
 def test_sliced_request_through_the_loop():
