@@ -328,11 +328,18 @@ int ome_server_set_request_limit(ome_server* srv, size_t bytes);
 typedef struct ome_server_stats {
   int64_t submitted;
   int64_t completed;
-  int64_t sliced;         /* submissions admitted as a sliced twin */
-  int64_t blocks_reused;  /* admissions lent the block the same plan used last */
-  int64_t kept_bytes;     /* retired blocks the loop keeps for later admissions */
+  int64_t sliced;           /* submissions admitted as a sliced twin */
+  int64_t blocks_reused;    /* admissions lent the block the same plan used last */
+  int64_t kept_bytes;       /* retired blocks the loop keeps for later admissions */
+  int64_t batched_launches; /* shared launches of several requests' fused small steps */
+  int64_t batched_requests; /* requests whose fused small steps ran in a shared launch */
 } ome_server_stats;
 int ome_server_get_stats(ome_server* srv, ome_server_stats* stats);
+/* Cross-request batching (on by default): the requests admitted together run the fused
+ * subtrees of small steps of their plans (unsliced plans, one element type) in one launch
+ * shared by all of them, then each its other steps.  Each request computes exactly what it
+ * computes alone. */
+int ome_server_set_batching(ome_server* srv, int on);
 
 #ifdef __cplusplus
 }

@@ -181,17 +181,16 @@ __global__ __launch_bounds__(256) void k_gemm_strided(StridedGemmDesc d) {
   }
 }
 
-template <class O>
-__global__ __launch_bounds__(SF_THREADS) void k_fused_steps(const FusedStep* __restrict__ steps,
-                                                            const int32_t* __restrict__ first,
-                                                            FusedLeaves lv, char* arena) {
+// One subtree of fused steps (steps [s0, s1) in order) by the whole workgroup; leaf(k) gives
+// the pointer of leaf k.
+template <class O, class Leaf>
+__device__ __forceinline__ void fused_subtree(const FusedStep* __restrict__ steps, int s0, int s1,
+                                              Leaf leaf, char* arena, int32_t* ska, int32_t* skb) {
   using T = typename O::T;
-  __shared__ int32_t ska[SF_MAXNK], skb[SF_MAXNK];
-  const int s1 = first[blockIdx.x + 1];
-  for (int s = first[blockIdx.x]; s < s1; ++s) {
+  for (int s = s0; s < s1; ++s) {
     const FusedStep& d = steps[s];
-    const T* __restrict__ A = d.a_leaf >= 0 ? (const T*)lv.p[d.a_leaf] : (const T*)(arena + d.a_off);
-    const T* __restrict__ B = d.b_leaf >= 0 ? (const T*)lv.p[d.b_leaf] : (const T*)(arena + d.b_off);
+    const T* __restrict__ A = d.a_leaf >= 0 ? (const T*)leaf(d.a_leaf) : (const T*)(arena + d.a_off);
+    const T* __restrict__ B = d.b_leaf >= 0 ? (const T*)leaf(d.b_leaf) : (const T*)(arena + d.b_off);
     T* __restrict__ Y = (T*)(arena + d.y_off);
     const int nk = d.nk;
     for (int k = threadIdx.x; k < nk; k += SF_THREADS) {
@@ -228,6 +227,30 @@ __global__ __launch_bounds__(SF_THREADS) void k_fused_steps(const FusedStep* __r
     // this step's outputs are the next steps' operands; the offset table is rewritten next
     __syncthreads();
   }
+}
+
+template <class O>
+__global__ __launch_bounds__(SF_THREADS) void k_fused_steps(const FusedStep* __restrict__ steps,
+                                                            const int32_t* __restrict__ first,
+                                                            FusedLeaves lv, char* arena) {
+  __shared__ int32_t ska[SF_MAXNK], skb[SF_MAXNK];
+  fused_subtree<O>(steps, first[blockIdx.x], first[blockIdx.x + 1],
+                   [&](int k) { return lv.p[k]; }, arena, ska, skb);
+}
+
+// Workgroup b runs subtree b - sub_begin of the job whose range holds b.
+template <class O>
+__global__ __launch_bounds__(SF_THREADS) void k_fused_jobs(const FusedJob* __restrict__ jobs,
+                                                           int njobs) {
+  __shared__ int32_t ska[SF_MAXNK], skb[SF_MAXNK];
+  const int b = blockIdx.x;
+  int j = 0;
+  while (j + 1 < njobs && jobs[j + 1].sub_begin <= b) ++j;
+  const FusedJob& jb = jobs[j];
+  const int q = b - jb.sub_begin;
+  const void* const* __restrict__ leaves = jb.leaves;
+  fused_subtree<O>(jb.steps, jb.first[q], jb.first[q + 1], [&](int k) { return leaves[k]; },
+                   jb.arena, ska, skb);
 }
 
 // One thread per output element, K walked directly from global memory: for tiny M/N or K
@@ -402,6 +425,16 @@ void launch_gemm_strided(int dt, const StridedGemmDesc& d, hipStream_t st) {
     StridedGemmDesc dd = d;
     dd.mode = epi_mode<O>(d.sx, d.sy);
     hipLaunchKernelGGL(k_gemm_strided<O>, dim3((uint32_t)g), dim3(256), 0, st, dd);
+  });
+}
+
+void launch_fused_jobs(int dt, const FusedJob* jobs, int njobs, int nsub_total, hipStream_t st) {
+  if (njobs <= 0 || nsub_total <= 0) return;
+  OME_DRY_RETURN(1);
+  TimedScope ts(KC_GEMM_VALU, st);
+  OME_DISPATCH(dt, O, {
+    hipLaunchKernelGGL(k_fused_jobs<O>, dim3((uint32_t)nsub_total), dim3(SF_THREADS), 0, st, jobs,
+                       njobs);
   });
 }
 

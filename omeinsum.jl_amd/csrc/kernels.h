@@ -106,6 +106,18 @@ struct FusedLeaves {
 };
 void launch_fused_steps(int dt, const FusedStep* steps, const int32_t* first, int nsub,
                         const FusedLeaves& leaves, char* arena, hipStream_t st);
+// The fused subtrees of several plans (several requests of the request loop) in one launch:
+// job j owns workgroups [sub_begin, sub_begin + nsub) and evaluates its plan's subtrees on its
+// own arena and leaves (a device table of leaf pointers).  jobs is a device array.
+struct FusedJob {
+  const FusedStep* steps;
+  const int32_t* first;
+  char* arena;
+  const void* const* leaves;
+  int32_t sub_begin, nsub;
+};
+constexpr int SF_MAXJOBS = 64;
+void launch_fused_jobs(int dt, const FusedJob* jobs, int njobs, int nsub_total, hipStream_t st);
 
 // Few outputs (M*N*batch <= 1024), long K: one workgroup per output and K part.
 int dot_plan_parts(const GemmDesc& d);

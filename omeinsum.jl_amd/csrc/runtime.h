@@ -156,7 +156,11 @@ Scal scal_zero(int dt);
 
 // Request-loop access to tree plans (tree.hip).
 int64_t tree_mem_bytes(const ome_tree* t);
-Status tree_run_lent(ome_tree* t, const void* const* xs, void* y, char* mem, hipStream_t st);
+// fused_done: the plan's fused subtrees already ran in a shared launch (tree_fused_job)
+Status tree_run_lent(ome_tree* t, const void* const* xs, void* y, char* mem, hipStream_t st,
+                     bool fused_done = false);
+bool tree_fused_job(const ome_tree* t, FusedJob* job);
+int tree_dtype(const ome_tree* t);
 void tree_return_lent(ome_tree* t);
 // The plan to admit under a per-request memory limit: t, or a sliced twin that fits.
 Status tree_fit(ome_tree* t, int64_t cap, ome_tree** out);

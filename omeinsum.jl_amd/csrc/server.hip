@@ -22,11 +22,15 @@
 // whose plan needs more than the per-request limit (the whole budget unless set) is sliced at
 // submission: the loop admits a twin of its tree with more labels sliced, whose memory fits
 // (SlicedEinsum semantics, FP64 cross-slice sum).
+// Cross-request batching: the requests admitted in one round run their fused small-step
+// subtrees (the maximal subtrees of small steps a plan evaluates in one launch) together, in
+// one launch shared by all of them, before each runs its other steps on its slot stream.
 // Sliced requests (SlicedEinsum plans) keep their FP64 cross-slice accumulator in the lent
 // memory.  A tree must not be used elsewhere while its request is queued or running, and the
 // context's stream must not be under graph capture while the loop has queued requests.
 #include <chrono>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <thread>
 #include <unordered_map>
@@ -65,7 +69,6 @@ struct Block {
 struct Slot {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
-  hipEvent_t admit = nullptr;  // recorded on the context's stream when a request is admitted
   Req* req = nullptr;
 };
 
@@ -78,6 +81,13 @@ struct ome_server {
   std::vector<Block> free_blocks;  // retired blocks kept for the next admissions
   size_t free_bytes = 0;
   ome_server_stats stats{};
+  // the shared launch of fused subtrees across requests
+  bool batching = true;
+  hipStream_t bst = nullptr;                 // batch stream
+  hipEvent_t round_admit = nullptr, batch_done = nullptr, copy_ev[2] = {nullptr, nullptr};
+  char* job_host[2] = {nullptr, nullptr};    // pinned staging of the job table, two buffers
+  char* job_dev[2] = {nullptr, nullptr};
+  int jb = 0;
   std::vector<Slot> slots;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -112,9 +122,104 @@ void retire(ome_server* s, Slot& sl, int status, const std::string& msg) {
   --s->active;
 }
 
+// Launch the requests of one admission round (their slots claimed, memory lent).  Every slot
+// stream waits for the submission event of its request and for the context's stream as it
+// stands now.  With batching on and two or more such requests in the round, the fused
+// small-step subtrees of the round's unsliced requests (one element type, at most SF_MAXJOBS
+// requests) run first in one launch on the batch stream, from a job table copied to the
+// device; those requests then run only their other steps, after that launch.  Returns one
+// status per request.
+std::vector<Status> launch_round(ome_server* s, const std::vector<Slot*>& round) {
+  std::vector<Status> res(round.size());
+  hipError_t ea;
+  {
+    std::lock_guard<std::recursive_mutex> g(s->ctx->mu);
+    ea = hipEventRecord(s->round_admit, s->ctx->stream);
+  }
+  if (ea != hipSuccess) {
+    hipGetLastError();
+    for (auto& r : res)
+      r = Status::err(OME_ERR_DEVICE, "could not order the request after the context's stream");
+    return res;
+  }
+  std::vector<char> batched(round.size(), 0);
+  if (s->batching) {
+    std::vector<FusedJob> jobs;
+    std::vector<size_t> who;
+    int dt = -1, nsub = 0;
+    for (size_t i = 0; i < round.size() && (int)jobs.size() < SF_MAXJOBS; ++i) {
+      Req* r = round[i]->req;
+      FusedJob jb{};
+      if (!tree_fused_job(r->tree, &jb)) continue;
+      if (dt < 0) dt = tree_dtype(r->tree);
+      if (tree_dtype(r->tree) != dt) continue;
+      jb.arena = r->mem;
+      jb.sub_begin = nsub;
+      nsub += jb.nsub;
+      jobs.push_back(jb);
+      who.push_back(i);
+    }
+    // one request alone gains nothing from the shared launch (its own graph holds its fused
+    // launch) and would wait for the job table's copy: batch rounds of two or more
+    if (jobs.size() >= 2) {
+      const int b = s->jb;
+      s->jb ^= 1;
+      bool ok = hipEventSynchronize(s->copy_ev[b]) == hipSuccess;  // staging b was copied
+      char* host = s->job_host[b];
+      char* dev = s->job_dev[b];
+      const size_t jbytes = SF_MAXJOBS * sizeof(FusedJob);
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        Req* r = round[who[q]]->req;
+        const void** lt = (const void**)(host + jbytes) + q * SF_MAXLEAF;
+        for (size_t k = 0; k < r->xs.size() && k < (size_t)SF_MAXLEAF; ++k) lt[k] = r->xs[k];
+        jobs[q].leaves = (const void* const*)(dev + jbytes) + q * SF_MAXLEAF;
+      }
+      std::memcpy(host, jobs.data(), jobs.size() * sizeof(FusedJob));
+      const size_t bytes = jbytes + jobs.size() * SF_MAXLEAF * sizeof(void*);
+      ok = ok && hipStreamWaitEvent(s->bst, s->round_admit, 0) == hipSuccess;
+      for (size_t q = 0; ok && q < jobs.size(); ++q)
+        ok = hipStreamWaitEvent(s->bst, round[who[q]]->req->ready, 0) == hipSuccess;
+      ok = ok && hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s->bst) == hipSuccess;
+      ok = ok && hipEventRecord(s->copy_ev[b], s->bst) == hipSuccess;
+      if (ok) {
+        launch_fused_jobs(dt, (const FusedJob*)dev, (int)jobs.size(), nsub, s->bst);
+        ok = hipGetLastError() == hipSuccess && hipEventRecord(s->batch_done, s->bst) == hipSuccess;
+      }
+      if (ok) {
+        for (size_t q : who) batched[q] = 1;
+        std::lock_guard<std::mutex> lk(s->mu);
+        ++s->stats.batched_launches;
+        s->stats.batched_requests += (int64_t)jobs.size();
+      } else {
+        hipGetLastError();
+        // the requests run their own fused launches; the batch stream is drained first so
+        // that no partial batch writes into their memory afterwards
+        hipStreamSynchronize(s->bst);
+        hipGetLastError();
+      }
+    }
+  }
+  for (size_t i = 0; i < round.size(); ++i) {
+    Slot& sl = *round[i];
+    Req* r = sl.req;
+    hipStreamWaitEvent(sl.st, r->ready, 0);
+    hipStreamWaitEvent(sl.st, s->round_admit, 0);
+    if (batched[i]) hipStreamWaitEvent(sl.st, s->batch_done, 0);
+    Status st = tree_run_lent(r->tree, r->xs.data(), r->y, r->mem, sl.st, batched[i] != 0);
+    const hipError_t e = hipEventRecord(sl.done, sl.st);
+    if (!st.good() || e != hipSuccess) {
+      if (e != hipSuccess) hipGetLastError();
+      hipStreamSynchronize(sl.st);
+      res[i] = st.good() ? Status::err(OME_ERR_DEVICE, "event record failed") : st;
+    }
+  }
+  return res;
+}
+
 void scheduler(ome_server* s) {
   hipSetDevice(s->ctx->device);
   std::unique_lock<std::mutex> lk(s->mu);
+  std::vector<Slot*> round;
   for (;;) {
     bool progress = false;
     // 1. retire finished requests
@@ -131,7 +236,7 @@ void scheduler(ome_server* s) {
       progress = true;
     }
     if (progress) s->cv_done.notify_all();
-    // 2. admit queued requests that fit, in arrival order
+    // 2. admit queued requests that fit, in arrival order: claim a slot and memory
     for (auto it = s->queue.begin(); it != s->queue.end();) {
       Slot* free_slot = nullptr;
       for (auto& sl : s->slots)
@@ -194,34 +299,21 @@ void scheduler(ome_server* s) {
       s->in_use += held;
       ++s->active;
       free_slot->req = r;
-      Slot& sl = *free_slot;
-      // enqueue outside the server lock (submitters keep going); the slot is claimed
+      round.push_back(free_slot);
+    }
+    // 3. launch the round outside the server lock (submitters keep going; the slots are
+    // claimed): the shared launch of the round's fused subtrees, then every request's steps
+    if (!round.empty()) {
       lk.unlock();
-      hipError_t ea;
-      {
-        std::lock_guard<std::recursive_mutex> g(s->ctx->mu);
-        ea = hipEventRecord(sl.admit, s->ctx->stream);
-      }
-      Status st = ea == hipSuccess
-                      ? Status::ok()
-                      : Status::err(OME_ERR_DEVICE, "could not order the request after the context's stream");
-      if (st.good()) {
-        hipStreamWaitEvent(sl.st, r->ready, 0);
-        hipStreamWaitEvent(sl.st, sl.admit, 0);
-        st = tree_run_lent(r->tree, r->xs.data(), r->y, mem, sl.st);
-      } else {
-        hipGetLastError();
-      }
-      hipError_t e = hipEventRecord(sl.done, sl.st);
+      std::vector<Status> res = launch_round(s, round);
       lk.lock();
-      if (!st.good() || e != hipSuccess) {
-        if (e != hipSuccess) hipGetLastError();
-        hipStreamSynchronize(sl.st);
-        retire(s, sl, -(st.good() ? OME_ERR_DEVICE : st.code), st.good() ? "event record failed" : st.msg);
+      for (size_t i = 0; i < round.size(); ++i) {
+        if (res[i].good()) continue;
+        retire(s, *round[i], -res[i].code, res[i].msg);
         s->cv_done.notify_all();
       }
+      round.clear();
       progress = true;
-      it = s->queue.begin();  // the queue may have changed while unlocked
     }
     if (s->stop && s->queue.empty() && s->active == 0) break;
     if (!progress) {
@@ -234,6 +326,31 @@ void scheduler(ome_server* s) {
       }
     }
   }
+}
+
+// Streams, events and buffers of a server whose scheduler is not running (destroy, or a
+// failed create): every stream is drained first.
+void free_server_resources(ome_server* s) {
+  for (auto& sl : s->slots) {
+    if (sl.st) {
+      hipStreamSynchronize(sl.st);
+      hipStreamDestroy(sl.st);
+    }
+    if (sl.done) hipEventDestroy(sl.done);
+  }
+  if (s->bst) {
+    hipStreamSynchronize(s->bst);
+    hipStreamDestroy(s->bst);
+  }
+  if (s->round_admit) hipEventDestroy(s->round_admit);
+  if (s->batch_done) hipEventDestroy(s->batch_done);
+  for (int b = 0; b < 2; ++b) {
+    if (s->copy_ev[b]) hipEventDestroy(s->copy_ev[b]);
+    if (s->job_host[b]) hipHostFree(s->job_host[b]);
+    if (s->job_dev[b]) s->ctx->pool->free(s->job_dev[b]);
+  }
+  for (auto& b : s->free_blocks) s->ctx->pool->free(b.p);
+  s->free_blocks.clear();
 }
 
 int sfail(ome_server* s, int code, const std::string& msg) {
@@ -256,20 +373,28 @@ int ome_server_create(ome_ctx* ctx, size_t hbm_budget, int max_active, ome_serve
   s->budget = hbm_budget;
   s->req_limit = hbm_budget;
   s->slots.resize(max_active);
-  for (auto& sl : s->slots) {
-    if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.admit, hipEventDisableTiming) != hipSuccess) {
-      hipGetLastError();
-      for (auto& q : s->slots) {
-        if (q.st) hipStreamDestroy(q.st);
-        if (q.done) hipEventDestroy(q.done);
-        if (q.admit) hipEventDestroy(q.admit);
-      }
-      delete s;
-      ctx->last_error = "could not create the request-loop streams";
-      return OME_ERR_DEVICE;
+  bool ok = true;
+  for (auto& sl : s->slots)
+    ok = ok && hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess;
+  const size_t jbytes = SF_MAXJOBS * (sizeof(FusedJob) + SF_MAXLEAF * sizeof(void*));
+  ok = ok && hipStreamCreateWithFlags(&s->bst, hipStreamNonBlocking) == hipSuccess &&
+       hipEventCreateWithFlags(&s->round_admit, hipEventDisableTiming) == hipSuccess &&
+       hipEventCreateWithFlags(&s->batch_done, hipEventDisableTiming) == hipSuccess;
+  for (int b = 0; ok && b < 2; ++b) {
+    ok = hipEventCreateWithFlags(&s->copy_ev[b], hipEventDisableTiming) == hipSuccess &&
+         hipHostMalloc((void**)&s->job_host[b], jbytes, hipHostMallocDefault) == hipSuccess;
+    if (ok) {
+      s->job_dev[b] = (char*)ctx->pool->alloc(jbytes);
+      ok = s->job_dev[b] != nullptr;
     }
+  }
+  if (!ok) {
+    hipGetLastError();
+    free_server_resources(s);
+    delete s;
+    ctx->last_error = "could not create the request-loop streams and buffers";
+    return OME_ERR_DEVICE;
   }
   s->th = std::thread(scheduler, s);
   *out = s;
@@ -361,18 +486,19 @@ int ome_server_destroy(ome_server* s) {
   s->cv_work.notify_all();
   s->th.join();
   hipSetDevice(s->ctx->device);
-  for (auto& sl : s->slots) {
-    hipStreamSynchronize(sl.st);
-    hipStreamDestroy(sl.st);
-    hipEventDestroy(sl.done);
-    hipEventDestroy(sl.admit);
-  }
   for (auto& kv : s->reqs) {
     hipEventDestroy(kv.second->ready);
     delete kv.second;
   }
-  for (auto& b : s->free_blocks) s->ctx->pool->free(b.p);
+  free_server_resources(s);
   delete s;
+  return OME_OK;
+}
+
+int ome_server_set_batching(ome_server* s, int on) {
+  if (!s) return OME_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->batching = on != 0;
   return OME_OK;
 }
 

@@ -99,6 +99,7 @@ struct TreeStep {
 
 struct GraphEntry {
   char* mem;  // the plan's memory when the graph was recorded (its own or lent)
+  bool fused_elsewhere;  // recorded without the fused subtrees (run by the request loop)
   std::vector<const void*> xs;
   void* y;
   Scal sx, sy;
@@ -145,6 +146,9 @@ struct ome_tree {
   // the request loop's admission: a sliced twin whose memory fits fit_cap bytes
   ome_tree* fit = nullptr;
   int64_t fit_cap = 0;
+  // set while the request loop evaluates the plan after running its fused subtrees in a
+  // launch shared with other requests: the evaluation leaves them out
+  bool fused_elsewhere = false;
 };
 
 namespace {
@@ -357,7 +361,7 @@ static Status run_steps(ome_tree* t, const std::vector<const void*>& xs, void* r
   const int n = (int)t->steps.size();
   // the fused subtrees first, in one launch (the normalised evaluation runs every step alone)
   const bool fuse = t->fsub > 0 && !normalize;
-  if (fuse) {
+  if (fuse && !t->fused_elsewhere) {
     FusedLeaves lv{};
     for (int k = 0; k < t->n_leaves; ++k) lv.p[k] = xs[k];
     launch_fused_steps(t->dt, t->fdev, t->ffirst, t->fsub, lv, t->mem, st);
@@ -510,8 +514,8 @@ static Status enqueue_eval(ome_tree* t, const std::vector<const void*>& xs, void
 static Status launch_graphed(ome_tree* t, const std::vector<const void*>& xv, void* y,
                              const Scal& sx, const Scal& sy, hipStream_t st) {
   auto same = [&](const GraphEntry& e) {
-    return e.mem == t->mem && e.xs == xv && e.y == y && std::memcmp(&e.sx, &sx, sizeof sx) == 0 &&
-           std::memcmp(&e.sy, &sy, sizeof sy) == 0;
+    return e.mem == t->mem && e.fused_elsewhere == t->fused_elsewhere && e.xs == xv && e.y == y &&
+           std::memcmp(&e.sx, &sx, sizeof sx) == 0 && std::memcmp(&e.sy, &sy, sizeof sy) == 0;
   };
   auto it = std::find_if(t->graphs.begin(), t->graphs.end(), same);
   if (it == t->graphs.end()) {
@@ -532,6 +536,7 @@ static Status launch_graphed(ome_tree* t, const std::vector<const void*>& xv, vo
     }
     GraphEntry ge;
     ge.mem = t->mem;
+    ge.fused_elsewhere = t->fused_elsewhere;
     ge.xs = xv;
     ge.y = y;
     ge.sx = sx;
@@ -1066,7 +1071,8 @@ int64_t tree_mem_bytes(const ome_tree* t) { return t->mem_bytes; }
 
 // Evaluate t once on st in memory lent by the caller (mem_bytes bytes): the plan's own
 // memory, if any, goes back to the pool and its graphs are dropped.  y <- result.
-Status tree_run_lent(ome_tree* t, const void* const* xs, void* y, char* mem, hipStream_t st) {
+Status tree_run_lent(ome_tree* t, const void* const* xs, void* y, char* mem, hipStream_t st,
+                     bool fused_done) {
   ome_ctx* ctx = t->ctx;
   std::lock_guard<std::recursive_mutex> g(ctx->mu);
   OptScope os(ctx, t->opt);
@@ -1083,10 +1089,24 @@ Status tree_run_lent(ome_tree* t, const void* const* xs, void* y, char* mem, hip
       return Status::err(OME_ERR_INVALID_ARG, "null input pointer");
   // a plan lent the same memory again (the loop hands blocks back to the plan that used them)
   // replays its recorded graph: no host planning of the steps
-  if (t->opt.use_graphs && t->cap_stream)
-    return launch_graphed(t, xv, y, scal_one(t->dt), scal_zero(t->dt), st);
-  return enqueue_eval(t, xv, y, scal_one(t->dt), scal_zero(t->dt), st);
+  t->fused_elsewhere = fused_done && t->fsub > 0 && t->slicing.empty();
+  Status r = (t->opt.use_graphs && t->cap_stream)
+                 ? launch_graphed(t, xv, y, scal_one(t->dt), scal_zero(t->dt), st)
+                 : enqueue_eval(t, xv, y, scal_one(t->dt), scal_zero(t->dt), st);
+  t->fused_elsewhere = false;
+  return r;
 }
+
+// The plan's fused subtrees as a job of a shared launch (unsliced plans that have them).
+bool tree_fused_job(const ome_tree* t, FusedJob* job) {
+  if (t->fsub <= 0 || !t->slicing.empty() || t->n_leaves > SF_MAXLEAF) return false;
+  job->steps = t->fdev;
+  job->first = t->ffirst;
+  job->nsub = t->fsub;
+  return true;
+}
+
+int tree_dtype(const ome_tree* t) { return t->dt; }
 
 // The plan the request loop admits for t under a per-request limit of cap bytes: t itself
 // when its memory fits; else a twin of the same tree with more labels sliced (kept with t and

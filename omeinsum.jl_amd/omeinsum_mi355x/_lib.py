@@ -46,7 +46,8 @@ class PoolStats(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", c_i64), ("completed", c_i64), ("sliced", c_i64),
-                ("blocks_reused", c_i64), ("kept_bytes", c_i64)]
+                ("blocks_reused", c_i64), ("kept_bytes", c_i64), ("batched_launches", c_i64),
+                ("batched_requests", c_i64)]
 
 
 class TreeInfo(ctypes.Structure):
@@ -106,6 +107,7 @@ SIGNATURES = {
     "ome_server_destroy": (c_int, [c_void_p]),
     "ome_server_set_request_limit": (c_int, [c_void_p, c_size_t]),
     "ome_server_get_stats": (c_int, [c_void_p, P(ServerStats)]),
+    "ome_server_set_batching": (c_int, [c_void_p, c_int]),
 }
 
 _lib = None

@@ -8,8 +8,11 @@
 A native scheduler thread admits queued requests, in arrival order, whenever an execution
 slot is free and the plan's device memory fits the remaining budget; each slot runs on its
 own stream, so admitted requests overlap on the device and requests that arrive later are
-admitted as earlier ones retire.  Latency is measured from submission to the moment the
-scheduler sees the request's completion event.
+admitted as earlier ones retire.  A plan above ``request_limit`` bytes (default: the budget)
+is sliced at submission until it fits; a plan lent the memory block it used before replays
+its recorded graph; requests admitted together run their fused small-step subtrees in one
+shared launch (``batching``, on by default).  Latency is measured from submission to the
+moment the scheduler sees the request's completion event.
 """
 import ctypes
 
@@ -21,7 +24,7 @@ from .tree import TreePlan, element_code
 
 class Server:
     def __init__(self, hbm_budget=8 << 30, max_active=8, backend=None, device=None,
-                 request_limit=None):
+                 request_limit=None, batching=True):
         import torch
         self.ctx = _lib.context(device)
         self.backend = backend if backend is not None else MI355XBackend()
@@ -30,6 +33,8 @@ class Server:
                                                   ctypes.byref(h)), self.ctx.h)
         self.h = h
         self.budget = int(hbm_budget)
+        if not batching:
+            _lib.check(self.ctx.lib.ome_server_set_batching(self.h, 0), self.ctx.h)
         if request_limit is not None:
             _lib.check(self.ctx.lib.ome_server_set_request_limit(self.h, int(request_limit)),
                        self.ctx.h)
@@ -77,7 +82,8 @@ class Server:
 
     def stats(self):
         """Counters of the loop: submitted, completed, sliced (admitted as a sliced twin),
-        blocks_reused (lent the block the same plan used last), kept_bytes."""
+        blocks_reused (lent the block the same plan used last), kept_bytes, batched_launches
+        and batched_requests (fused small steps of several requests run in one launch)."""
         st = _lib.ServerStats()
         _lib.check(self.ctx.lib.ome_server_get_stats(self.h, ctypes.byref(st)), self.ctx.h)
         return {k: getattr(st, k) for k, _ in st._fields_}

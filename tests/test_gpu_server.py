@@ -184,3 +184,28 @@ def test_admission_waits_for_reused_pool_memory():
         ref0 = fp64(tree0, sd0, xs0)
         got0 = complex(om().to_numpy(y0)[()])
         assert abs(got0 - ref0) <= 1e-4 * abs(ref0), (got0, ref0)
+
+
+def test_cross_request_batching_matches_unbatched():
+    # requests admitted in one round run their fused small-step subtrees in one shared launch;
+    # every result equals the same request's result with batching off, bit for bit
+    from omeinsum_mi355x.server import Server
+    reqs = [request(n, 5600 + i) for i, n in enumerate([20, 24, 28, 30, 32, 36, 40, 44, 26, 22,
+                                                         34, 38])]
+    outs = {}
+    for batching in (False, True):
+        with Server(hbm_budget=1 << 30, max_active=8, batching=batching) as srv:
+            plans = [srv.plan(tree, dxs, sd) for tree, sd, xs, dxs in reqs]
+            for _ in range(2):  # the second pass replays graphs recorded without fused steps
+                tickets = [srv.submit(p, r[3]) for p, r in zip(plans, reqs)]
+                outs[batching] = [complex(om().to_numpy(srv.wait(t)[0])[()]) for t in tickets]
+            st = srv.stats()
+        if batching:
+            assert st["batched_launches"] >= 1
+            assert st["batched_requests"] > st["batched_launches"]
+        else:
+            assert st["batched_launches"] == 0
+    assert outs[True] == outs[False]
+    for (tree, sd, xs, _), v in zip(reqs, outs[True]):
+        ref = fp64(tree, sd, xs)
+        assert abs(v - ref) <= 1e-5 * abs(ref), (v, ref)
