@@ -35,4 +35,6 @@ def __getattr__(name):
         raise AttributeError(name)
     import importlib
     m = importlib.import_module("." + mod, __name__)
-    return getattr(m, name)
+    value = getattr(m, name)
+    globals()[name] = value  # later lookups are plain attribute reads
+    return value

@@ -173,7 +173,9 @@ class Context:
     def sync_stream(self):
         """Follow torch's current stream (the pool reuses blocks in stream order)."""
         import torch
-        s = torch.cuda.current_stream(self.device).cuda_stream
+        raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+        s = raw_stream(self.device) if raw_stream is not None else \
+            torch.cuda.current_stream(self.device).cuda_stream
         if s != self.stream:
             check(self.lib.ome_set_stream(self.h, c_void_p(s)), self.h)
             self.stream = s
@@ -221,7 +223,7 @@ _contexts = {}
 def context(device=None):
     import torch
     if device is None:
-        if not torch.cuda.is_available():
+        if not _contexts and not torch.cuda.is_available():
             raise RuntimeError("omeinsum_mi355x needs an MI355X (gfx950) GPU; none is visible")
         device = torch.cuda.current_device()
     ctx = _contexts.get(device)

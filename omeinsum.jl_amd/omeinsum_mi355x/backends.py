@@ -11,6 +11,7 @@
   GEMM-shaped steps use ``precision`` ("fp32", "bf16x3" or "fp64").
 """
 from . import _lib
+from . import ein_ops as E
 from .codes import EinCode, NestedEinsum, SlicedEinsum, drop_slicedim, get_size_dict
 
 
@@ -35,7 +36,6 @@ class DefaultBackend(EinsumBackend):
 
     # -- flat codes
     def einsum(self, code, xs, size_dict, **kw):
-        from . import ein_ops as E
         self.apply(_lib.context())
         if isinstance(code, EinCode):
             return E.flat_einsum(code, xs, size_dict)
@@ -44,7 +44,6 @@ class DefaultBackend(EinsumBackend):
         return self._nested(code, xs, size_dict)
 
     def einsum_inplace(self, code, xs, y, sx, sy, size_dict):
-        from . import ein_ops as E
         self.apply(_lib.context())
         if isinstance(code, EinCode):
             return E._flat_einsum_inplace(code.ixs, code.iy, xs, y, sx, sy, size_dict)
@@ -57,7 +56,6 @@ class DefaultBackend(EinsumBackend):
 
     def _nested(self, ne, xs, size_dict):
         """einsum(::NestedEinsum, xs, size_dict) (src/einsequence.jl:279-287)."""
-        from . import ein_ops as E
         mxs = [xs[a.tensorindex] if a.isleaf() else self._nested(a, xs, size_dict) for a in ne.args]
         return E.flat_einsum(ne.eins, tuple(mxs), size_dict)
 
@@ -65,7 +63,6 @@ class DefaultBackend(EinsumBackend):
         """einsum(::SlicedEinsum, xs, size_dict) (src/slicing.jl:123-137): zero output, loop
         the slices column-major over the sliced labels, take_slice, evaluate, fill_slice!."""
         import itertools
-        from . import ein_ops as E
         from .device import kind_of, raw, wrap
         if not se.slicing:
             return self.einsum(se.eins, xs, size_dict)
@@ -128,7 +125,6 @@ class MI355XBackend(EinsumBackend):
         ctx.set_option(_lib.OPT_DIRECT_GEMM, 1 if self.direct_gemm else 0)
 
     def einsum(self, code, xs, size_dict, **kw):
-        from . import ein_ops as E
         self.apply(_lib.context())
         if isinstance(code, EinCode):
             return E.flat_einsum(code, xs, size_dict)
@@ -137,7 +133,6 @@ class MI355XBackend(EinsumBackend):
         return plan.execute(xs, ts=ts)
 
     def einsum_inplace(self, code, xs, y, sx, sy, size_dict):
-        from . import ein_ops as E
         self.apply(_lib.context())
         if isinstance(code, EinCode):
             return E._flat_einsum_inplace(code.ixs, code.iy, xs, y, sx, sy, size_dict)

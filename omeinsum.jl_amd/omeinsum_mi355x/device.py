@@ -208,6 +208,9 @@ def _torch():
     return torch
 
 
+_PLAIN_CODES = {}  # torch dtype -> C ABI element type of plain tensors
+
+
 def dtype_code(t, kind=None):
     """C ABI element type of torch dtype t for plain tensors (kind None) or for the data of a
     wrapper class (a True kind means Tropical)."""
@@ -219,9 +222,12 @@ def dtype_code(t, kind=None):
             return kind.codes()[t]
         except KeyError:
             raise TypeError("%s tensors of %s are not supported" % (kind.__name__, t))
+    if not _PLAIN_CODES:
+        _PLAIN_CODES.update({torch.float32: _lib.F32, torch.float64: _lib.F64,
+                             torch.complex64: _lib.C32, torch.complex128: _lib.C64,
+                             torch.int64: _lib.I64})
     try:
-        return {torch.float32: _lib.F32, torch.float64: _lib.F64, torch.complex64: _lib.C32,
-                torch.complex128: _lib.C64, torch.int64: _lib.I64}[t]
+        return _PLAIN_CODES[t]
     except KeyError:
         raise TypeError("element type %s is not supported by the MI355X backend" % t)
 

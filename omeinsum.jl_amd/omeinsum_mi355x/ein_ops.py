@@ -10,7 +10,7 @@ import numpy as np
 
 from . import _lib
 from .codes import EinCode, NestedEinsum, SlicedEinsum, get_size_dict
-from .device import common_kind, dtype_code, kind_of, raw, scalar, to_device, wrap
+from .device import _AS_IS, common_kind, dtype_code, kind_of, raw, scalar, to_device, wrap
 
 
 def _torch():
@@ -21,6 +21,14 @@ def _torch():
 def _unwrap(xs):
     """-> (device tensors, wrapper class of the elements or None, common torch dtype)."""
     torch = _torch()
+    # plain contiguous tensors of one supported type on the current device pass as they are
+    t0 = xs[0] if xs else None
+    if type(t0) is torch.Tensor:
+        T, dev = t0.dtype, t0.device
+        if dev.type == "cuda" and dev.index == torch.cuda.current_device() and T in _AS_IS and \
+                all(type(x) is torch.Tensor and x.dtype == T and x.device == dev and
+                    x.is_contiguous() for x in xs):
+            return list(xs), None, T
     kind = common_kind(xs)
     ts = [to_device(raw(x)) for x in xs]
     T = ts[0].dtype
@@ -42,15 +50,21 @@ def _labels(ixs, iy):
 
 _ARGS = collections.OrderedDict()  # (ixs, iy, sizes) -> the ctypes label / size arrays
 _ARGS_MAX = 256
+_IDS = {}  # (ixs, iy) -> labels in first-appearance order
 
 
 def _flat_args(ixs, iy, size_dict):
     """(nix, cixs, ciy, nlabels, sizes) ctypes arrays of one flat code, cached per code and
     sizes (the arrays are read-only inputs of the C call)."""
-    key = (tuple(tuple(ix) for ix in ixs), tuple(iy))
-    ids = _labels(ixs, iy)
+    key = (tuple(map(tuple, ixs)), tuple(iy))
+    ids = _IDS.get(key)
+    if ids is None:
+        ids = _labels(ixs, iy)
+        if len(_IDS) > 4 * _ARGS_MAX:
+            _IDS.clear()
+        _IDS[key] = ids
     try:
-        key = key + (tuple(int(size_dict[l]) for l in ids),)
+        key = key + (tuple([int(size_dict[l]) for l in ids]),)
     except KeyError as e:
         raise KeyError("size of label %r is unknown (pass size_info)" % (e.args[0],))
     hit = _ARGS.get(key)
@@ -70,17 +84,37 @@ def _flat_args(ixs, iy, size_dict):
     return hit
 
 
+_PTR_ARRAYS = {}   # n -> ctypes array type of n pointers
+_SCALARS = {}      # (type, value, element code) -> ome_scalar (a read-only input)
+
+
+def _scalar(v, code):
+    key = (type(v), v, code)
+    try:
+        hit = _SCALARS.get(key)
+    except TypeError:  # unhashable number type
+        return scalar(v, code)
+    if hit is None:
+        hit = scalar(v, code)
+        if len(_SCALARS) > 1024:
+            _SCALARS.clear()
+        _SCALARS[key] = hit
+    return hit
+
+
 def _call_flat(ixs, iy, ts, yt, code, sx, sy, size_dict, force_loop=False):
     """One flat einsum on the device.  Row-major tensors are passed with reversed label
     lists (see device.py)."""
     ctx = _lib.context(yt.device.index)
     nix, cixs, ciy, nlabels, sizes = _flat_args(ixs, iy, size_dict)
     n = len(ixs)
-    ptrs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts])
-    s_x, s_y = scalar(sx, code), scalar(sy, code)
+    arr = _PTR_ARRAYS.get(n)
+    if arr is None:
+        arr = _PTR_ARRAYS[n] = ctypes.c_void_p * n
+    ptrs = arr(*[t.data_ptr() for t in ts])
     fn = ctx.lib.ome_loop_einsum if force_loop else ctx.lib.ome_einsum
-    _lib.check(fn(ctx.h, code, n, ptrs, nix, cixs, ctypes.c_void_p(yt.data_ptr()), len(iy), ciy,
-                  nlabels, sizes, ctypes.byref(s_x), ctypes.byref(s_y)), ctx.h)
+    _lib.check(fn(ctx.h, code, n, ptrs, nix, cixs, yt.data_ptr(), len(iy), ciy, nlabels, sizes,
+                  ctypes.byref(_scalar(sx, code)), ctypes.byref(_scalar(sy, code))), ctx.h)
 
 
 def _flat_einsum_inplace(ixs, iy, xs, y, sx, sy, size_dict, force_loop=False):
@@ -114,24 +148,33 @@ def flat_einsum(code, xs, size_dict):
     return _flat_einsum_inplace(code.ixs, code.iy, xs, y, True, False, size_dict)
 
 
+_GET_BACKEND = []
+
+
+def _active_backend():
+    """get_einsum_backend(), with backends imported once (it imports this module)."""
+    if not _GET_BACKEND:
+        from .backends import get_einsum_backend
+        _GET_BACKEND.append(get_einsum_backend)
+    return _GET_BACKEND[0]()
+
+
 def einsum(code, xs, size_dict=None, **kw):
     """einsum(code, xs, size_dict): EinCode, NestedEinsum or SlicedEinsum, dispatched on the
     active backend (src/backends.jl:108,219)."""
-    from .backends import get_einsum_backend
     xs = tuple(xs)
     if size_dict is None:
         size_dict = get_size_dict(code.getixsv(), xs)
-    return get_einsum_backend().einsum(code, xs, size_dict, **kw)
+    return _active_backend().einsum(code, xs, size_dict, **kw)
 
 
 def einsum_inplace(code, xs, y, sx=True, sy=False, size_dict=None):
     """einsum!(code, xs, y, sx, sy, size_dict) (src/einsum.jl:43-50, src/einsequence.jl:270-277,
     src/slicing.jl:112-121)."""
-    from .backends import get_einsum_backend
     xs = tuple(xs)
     if size_dict is None:
         size_dict = get_size_dict(code.getixsv(), xs)
-    return get_einsum_backend().einsum_inplace(code, xs, y, sx, sy, size_dict)
+    return _active_backend().einsum_inplace(code, xs, y, sx, sy, size_dict)
 
 
 def loop_einsum(code, xs, size_dict=None):
