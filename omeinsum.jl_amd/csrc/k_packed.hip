@@ -72,6 +72,14 @@ __device__ __forceinline__ const float* pack_addr(const PackDesc& d, const float
   return base + 2 * (nn * d.sr + kk * d.sk) + (kim != nim ? 1 : 0);
 }
 
+// Swizzle of the k-major staging tile: float4 group g of rows at k is stored at group
+// g ^ pk_swz(k).  The four emitter lanes of one row read k 8 apart (same bank without the
+// swizzle); xor-ing bits 2-3 of the group with k / 8 moves them 16 banks apart, so the 64
+// lanes of a wave reach every bank equally.  Within a row of groups the xor is a permutation,
+// so the float4 stores stay contiguous.
+__device__ __forceinline__ int pk_swz(int k) { return ((k >> 3) & 3) << 2; }
+static_assert(PK_K == 32, "the staging tile holds 32 k per row");
+
 // TR rows per packed tile (128 or 256).
 template <int TR>
 __device__ __forceinline__ void pack_tile(const PackDesc& d, int64_t bid, float (*tile)[PK_K + 1]) {
@@ -85,7 +93,12 @@ __device__ __forceinline__ void pack_tile(const PackDesc& d, int64_t bid, float 
   // TR*32/1024 float4 loads per thread, all issued before the LDS stores
   constexpr int NV = TR * PK_K / 1024;
   const bool full = d.mode == 0 && r0 + TR <= d.R && k0 + PK_K <= d.K;
-  if (full && d.rfast && d.sr == 1 && (d.sk % 4) == 0 && (((uintptr_t)base) & 15) == 0) {
+  // the row-fast fast path stages the tile k-major, tt[k][row] (same LDS), with the float4
+  // groups of rows swizzled by bit 3 of k: its float4 stores and the emitter's reads along k
+  // are then free of bank conflicts (a row-major tile took 8-way conflicted scalar stores)
+  float(*tt)[TR] = (float(*)[TR])&tile[0][0];
+  const bool kmaj = full && d.rfast && d.sr == 1 && (d.sk % 4) == 0 && (((uintptr_t)base) & 15) == 0;
+  if (kmaj) {
     float4 v[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
@@ -96,11 +109,8 @@ __device__ __forceinline__ void pack_tile(const PackDesc& d, int64_t bid, float 
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       const int e = t + 256 * i;
-      const int r4 = (e % (TR / 4)) * 4, k = e / (TR / 4);
-      tile[r4][k] = v[i].x;
-      tile[r4 + 1][k] = v[i].y;
-      tile[r4 + 2][k] = v[i].z;
-      tile[r4 + 3][k] = v[i].w;
+      const int g = e % (TR / 4), k = e / (TR / 4);
+      *(float4*)&tt[k][4 * (g ^ pk_swz(k))] = v[i];
     }
   } else if (full && d.sk == 1 && (d.sr % 4) == 0 && (((uintptr_t)base) & 15) == 0) {
     float4 v[NV];
@@ -155,7 +165,8 @@ __device__ __forceinline__ void pack_tile(const PackDesc& d, int64_t bid, float 
     uint16_t h[8], l[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const float x = tile[r][kq + e];
+      const int k = kq + e;
+      const float x = kmaj ? tt[k][4 * ((r >> 2) ^ pk_swz(k)) + (r & 3)] : tile[r][k];
       h[e] = f2bf_rn(x);
       l[e] = f2bf_rn(x - __uint_as_float((uint32_t)h[e] << 16));
     }

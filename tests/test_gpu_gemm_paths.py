@@ -168,7 +168,14 @@ def test_fp64_precision_rounds_once(dtype):
 
 LONG_K = [("ij,ij->", {"i": 256, "j": 300}),            # one output, K = 76800
           ("ijk,jkl->il", {"i": 3, "j": 100, "k": 100, "l": 5}),  # 15 outputs, K = 10^4
-          ("abc,abd->cd", {"a": 64, "b": 64, "c": 2, "d": 3})]    # 6 outputs, K = 4096
+          ("abc,abd->cd", {"a": 64, "b": 64, "c": 2, "d": 3}),    # 6 outputs, K = 4096
+          # the one-launch panel kernel: 1024 < outputs <= 16384, K >= 512
+          ("ijk,jkl->il", {"i": 64, "j": 64, "k": 64, "l": 64}),  # 64^3 inputs, 4096 outputs
+          ("ijk,jkl->il", {"i": 30, "j": 20, "k": 30, "l": 37}),  # ragged 4x4 tiles
+          ("ij,jk->ik", {"i": 2, "j": 1000, "k": 700}),            # M < 4: 1 x 16 tiles
+          ("ij,jk->ik", {"i": 900, "j": 700, "k": 3}),             # N < 4: 16 x 1 tiles
+          ("ijb,jkb->ikb", {"i": 9, "j": 600, "k": 10, "b": 13}),  # batched
+          ("aib,abk->ik", {"a": 16, "b": 40, "i": 37, "k": 33})]   # strided operands
 
 
 @pytest.mark.parametrize("spec,sd", LONG_K)
@@ -253,3 +260,32 @@ def test_c4_shaped_gradient_finite_differences():
         o.set_einsum_backend(o.DefaultBackend())
     assert abs(fa - np.sum(ga * va)) <= 1e-7 * max(1.0, abs(fa))
     assert abs(fb - np.sum(gb * vb)) <= 1e-7 * max(1.0, abs(fb))
+
+
+def test_one_launch_split_k_is_deterministic_across_calls():
+    """Few outputs / long K with M, N >= 16 run split over K in one launch, the last
+    workgroup of each tile summing the partials in split order: repeated calls, interleaved
+    with calls of other tile counts that share the stream's ticket counters, give bit-equal
+    results, within the FP32 bound of the FP64 oracle."""
+    o = om()
+    cases = [("ijk,jkl->il", {"i": 64, "j": 64, "k": 64, "l": 64}),
+             ("ijk,jkl->il", {"i": 30, "j": 20, "k": 30, "l": 37}),
+             ("aib,abk->ik", {"a": 16, "b": 40, "i": 37, "k": 33})]
+    r = rng(2107)
+    data = []
+    for spec, sd in cases:
+        ixs, iy = parse(spec)
+        xs = [rand(r, [sd[l] for l in ix], np.float32) for ix in ixs]
+        data.append((ixs, iy, xs, sd, [dev(x) for x in xs]))
+    first = {}
+    for rep in range(4):
+        for c, (ixs, iy, xs, sd, dxs) in enumerate(data):
+            y = o.einsum(o.EinCode(ixs, iy), dxs, sd)
+            torch.cuda.synchronize()
+            got = o.to_numpy(y)
+            if rep == 0:
+                first[c] = got
+                k = int(np.prod([sd[l] for l in set(ixs[0]) & set(ixs[1]) - set(iy)]))
+                assert_close(got, oracle(ixs, iy, xs, sd), np.float32, k, str(c))
+            else:
+                assert np.array_equal(got, first[c]), (c, rep)
