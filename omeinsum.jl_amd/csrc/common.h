@@ -253,6 +253,21 @@ struct OpsTBool {
     default: break;                                         \
   }
 
+// BF16x3 split of two floats on gfx950's packed float -> bfloat16 conversion (round to
+// nearest even): hi = the two bf16(x) (a in the low half), lo = the two bf16(x - hi).
+typedef float split_f2 __attribute__((ext_vector_type(2)));
+typedef __bf16 split_b2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split2_bf16(float a, float b, uint32_t& hi, uint32_t& lo) {
+  const split_f2 x = {a, b};
+  const split_b2 h = __builtin_convertvector(x, split_b2);
+  uint32_t hb;
+  __builtin_memcpy(&hb, &h, 4);
+  const split_f2 hf = {__uint_as_float(hb << 16), __uint_as_float(hb & 0xffff0000u)};
+  const split_b2 l = __builtin_convertvector(x - hf, split_b2);
+  __builtin_memcpy(&lo, &l, 4);
+  hi = hb;
+}
+
 // Epilogue mode:  y <- sx*v (sy zero: y not read) | sy*y + sx*v | sy*y (sx zero).
 enum EpiMode : int { EPI_OVERWRITE = 0, EPI_AXPBY = 1, EPI_SCALE_ONLY = 2 };
 

@@ -31,6 +31,8 @@ struct GroupEntry {
   int64_t total = 0;
   int64_t split = 0;  // grouped permute: blocks of the shared-dim problems, which come first
   bool pinned = false;  // used under graph capture: kept for the context's life
+  int64_t xfirst[GG_XCDS + 1] = {};  // grouped GEMM: first tile of each XCD's range
+  bool offsets32 = false;  // grouped GEMM: every problem fits 32-bit buffer offsets
 };
 
 inline int ilog2(double v) {
@@ -165,20 +167,43 @@ static GroupEntry* gg_entry(ome_ctx* ctx, hipStream_t st, int dt, int variant,
   const size_t bytes = pbytes + fbytes + std::max<int64_t>(ntiles, 1) * sizeof(int32_t);
   return group_entry(ctx, st, key, bytes, [&](char* h, GroupEntry& e) {
     // longest K first, then the larger output: tiles of similar cost sit together
-    std::vector<int> order(ps.size());
-    for (size_t i = 0; i < ps.size(); ++i) order[i] = (int)i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    std::vector<int> sorted(ps.size());
+    for (size_t i = 0; i < ps.size(); ++i) sorted[i] = (int)i;
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int x, int y) {
       if (ps[x].k != ps[y].k) return ps[x].k > ps[y].k;
       return ps[x].m * ps[x].n > ps[y].m * ps[y].n;
     });
+    // deal the problems in that order to the least loaded XCD (cost: tiles x (K + a fixed
+    // per-tile share)); each XCD's problems keep the order, and its tiles form one range
+    std::vector<int> per_xcd[GG_XCDS];
+    double load[GG_XCDS] = {};
+    for (int i : sorted) {
+      const GGProblem& p = ps[i];
+      if (p.m <= 0 || p.n <= 0 || p.batch <= 0) continue;
+      const double nt = (double)((p.m + tm - 1) / tm) * ((p.n + tn - 1) / tn) * p.batch;
+      int best = 0;
+      for (int c = 1; c < GG_XCDS; ++c)
+        if (load[c] < load[best]) best = c;
+      load[best] += nt * (double)(std::max<int64_t>(p.k, 1) + 32);
+      per_xcd[best].push_back(i);
+    }
+    std::vector<int> order;
+    order.reserve(ps.size());
+    std::vector<size_t> xcd_np(GG_XCDS + 1, 0);
+    for (int c = 0; c < GG_XCDS; ++c) {
+      xcd_np[c] = order.size();
+      order.insert(order.end(), per_xcd[c].begin(), per_xcd[c].end());
+    }
+    xcd_np[GG_XCDS] = order.size();
     GGProb* probs = (GGProb*)h;
     int64_t* first = (int64_t*)(h + pbytes);
     int32_t* map = (int32_t*)(h + pbytes + fbytes);
     int np = 0;
     int64_t total = 0;
-    for (int i : order) {
-      const GGProblem& p = ps[i];
-      if (p.m <= 0 || p.n <= 0 || p.batch <= 0) continue;
+    int xc = 0;
+    for (size_t oi = 0; oi < order.size(); ++oi) {
+      while (xc < GG_XCDS && oi == xcd_np[xc]) e.xfirst[xc++] = total;
+      const GGProblem& p = ps[order[oi]];
       GGProb g{};
       g.a = p.a;
       g.b = p.b;
@@ -196,6 +221,15 @@ static GroupEntry* gg_entry(ome_ctx* ctx, hipStream_t st, int dt, int variant,
       for (int64_t q = 0; q < nt; ++q) map[total + q] = np;
       ++np;
       total += nt;
+    }
+    while (xc <= GG_XCDS) e.xfirst[xc++] = total;
+    e.offsets32 = true;
+    for (const auto& p : ps) {
+      if (p.m <= 0 || p.n <= 0 || p.batch <= 0) continue;
+      const int64_t lim = (int64_t)1 << 22, slim = (int64_t)1 << 31;
+      if (p.m >= lim || p.n >= lim || p.k >= lim || p.m * p.k * 4 >= slim ||
+          p.k * p.n * 4 >= slim || p.m * p.n * 4 >= slim)
+        e.offsets32 = false;
     }
     e.first_off = pbytes;
     e.map_off = pbytes + fbytes;
@@ -218,6 +252,7 @@ static std::vector<GGCand> gg_candidates(int dt, int precision, double mean_mn) 
     c.push_back({GG_BF16X3_W32, 0});
     c.push_back({GG_BF16X3_64, 0});
     c.push_back({GG_BF16X3_64, 1});
+    c.push_back({GG_BF16X3_64_BK64, 0});
     if (mean_mn >= 96) c.push_back({GG_BF16X3_128, 0});
   } else if (dt == F32) {
     c.push_back({GG_F32_W32, 1});
@@ -233,14 +268,21 @@ static std::vector<GGCand> gg_candidates(int dt, int precision, double mean_mn) 
   return c;
 }
 
-static int gg_grid(ome_ctx* ctx, int dt, const GGCand& c, int64_t total) {
+// A multiple of GG_XCDS: every XCD gets the same number of workgroups for its range.
+static int gg_grid(ome_ctx* ctx, int dt, const GGCand& c, const GroupEntry& e) {
   // wave-per-tile variants: four tiles per workgroup
   const bool wave = dt == F32 && (c.variant == GG_F32_W32 || c.variant == GG_BF16X3_W32);
-  const int64_t groups = wave ? (total + 3) / 4 : total;
-  if (!c.persistent) return (int)std::min<int64_t>(groups, 1 << 30);
-  const int per_cu = (dt == F32 && (c.variant == GG_F32_128 || c.variant == GG_BF16X3_128)) ? 2
-                     : wave ? 8 : 4;
-  return (int)std::min<int64_t>(groups, (int64_t)ctx->info.compute_units * per_cu);
+  int64_t widest = 0;
+  for (int x = 0; x < GG_XCDS; ++x) widest = std::max(widest, e.xfirst[x + 1] - e.xfirst[x]);
+  int64_t per = wave ? (widest + 3) / 4 : widest;  // workgroups one XCD's range can use
+  if (per < 1) per = 1;
+  if (c.persistent) {
+    const int per_cu = (dt == F32 && (c.variant == GG_F32_128 || c.variant == GG_BF16X3_128)) ? 2
+                       : wave ? 8 : 4;
+    per = std::min<int64_t>(per, std::max<int64_t>(1, (int64_t)ctx->info.compute_units * per_cu /
+                                                          GG_XCDS));
+  }
+  return (int)std::min<int64_t>(per * GG_XCDS, (int64_t)1 << 30);
 }
 
 static Status gg_launch(ome_ctx* ctx, hipStream_t st, int dt, const GGCand& c,
@@ -259,7 +301,9 @@ static Status gg_launch(ome_ctx* ctx, hipStream_t st, int dt, const GGCand& c,
   g.alpha = alpha;
   g.beta = beta;
   g.mode = dt == F32 ? epi_mode<OpsF32>(alpha, beta) : 0;
-  launch_grouped_gemm(dt, c.variant, g, gg_grid(ctx, dt, c, e->total), st);
+  for (int x = 0; x <= GG_XCDS; ++x) g.xcd_first[x] = e->xfirst[x];
+  g.offsets32 = e->offsets32 ? 1 : 0;
+  launch_grouped_gemm(dt, c.variant, g, gg_grid(ctx, dt, c, *e), st);
   return Status::ok();
 }
 

@@ -2,10 +2,11 @@
 //
 // Grouped GEMM: problem g computes C_g[:,:,l] = alpha*A_g[:,:,l]*B_g[:,:,l] + beta*C_g[:,:,l]
 // (column-major, dense, no transposes; A m x k, B k x n).  Every problem is cut into output
-// tiles; the tiles of all problems form one list, ordered by problem (the host sorts the
-// problems by K, longest first, so tiles of similar cost are adjacent).  A fixed grid of
-// workgroups walks the list round-robin (tile = blockIdx.x + i*gridDim.x) and finds the
-// problem of a tile in a tile -> problem table (one load, not a search).
+// tiles; the tiles of all problems form one list, ordered by problem and cut into one range
+// per XCD (the host deals the problems, longest K first, to the least loaded XCD, so each
+// range holds whole problems of every cost).  The workgroups of XCD c walk its range
+// round-robin and find the problem of a tile in a tile -> problem table (one load, not a
+// search); a problem's operand panels are then read into that XCD's L2 only.
 //
 //   Float32:          MFMA v_mfma_f32_32x32x2_f32, workgroup tile TM x TN (4 waves, 2x2),
 //                     K step BK staged through LDS with a register prefetch of the next step;
@@ -91,6 +92,81 @@ __device__ __forceinline__ uint32_t gg_load_b(const float* B, int64_t K, int64_t
   }
   return mask;
 }
+// Buffer resources over one operand slice (raw: no stride, range-checked against `bytes`).
+// Offsets at or past GG_OOB lie outside every slice the host admits (< 2^31 bytes).
+typedef uint32_t gg_u4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t GG_OOB = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t gg_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           0x00020000);
+}
+
+// A: EPT consecutive k (from k) of row m through rs; a k past K lands past the slice (zero),
+// and a row past M starts past it and stays there.
+template <int EPT>
+__device__ __forceinline__ void gg_buf_load_a(__amdgpu_buffer_rsrc_t rs, bool mok, int64_t m,
+                                              int64_t k, int64_t M, float (&ra)[EPT]) {
+  uint32_t off = mok ? (uint32_t)(m + k * M) * 4u : GG_OOB;
+  const uint32_t inc = mok ? (uint32_t)M * 4u : 0u;
+#pragma unroll
+  for (int e = 0; e < EPT; ++e, off += inc)
+    ra[e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0));
+}
+// B: the float4 groups of gg_load_b (4 k at one n) through rs; a column past N lands past the
+// slice, and a k past K is sent past it explicitly (it would reach the next column).
+template <int NQ, int BK>
+__device__ __forceinline__ void gg_buf_load_b(__amdgpu_buffer_rsrc_t rs, bool vec, int64_t K,
+                                              int64_t n0, int64_t k0, float4 (&rb)[NQ]) {
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int e = threadIdx.x + 256 * q;
+    const int64_t k = k0 + (e % (BK / 4)) * 4, n = n0 + e / (BK / 4);
+    if (vec) {  // K % 4 == 0: a float4 is wholly inside or wholly outside
+      const uint32_t o = k < K ? (uint32_t)(k + n * K) * 4u : GG_OOB;
+      const gg_u4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)o, 0, 0);
+      rb[q] = make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
+                          __uint_as_float(v.w));
+    } else {
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t o = k + i < K ? (uint32_t)(k + i + n * K) * 4u : GG_OOB;
+        v[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)o, 0, 0));
+      }
+      rb[q] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+  }
+}
+// C: the 32x32 MFMA blocks of a wave (lane: row m, columns 8 (q / 4) + 4 (lane / 32) + q % 4)
+// through a resource over the C slice; a column past N lands past it (the store is dropped),
+// a row past M is sent past it explicitly.
+template <int NB, int MB>
+__device__ __forceinline__ void gg_buf_store_c(float* C, int64_t M, int64_t N, int64_t m0,
+                                               int64_t n0, int lane, int mode, float alpha,
+                                               float beta, const floatx16 (&acc)[NB][MB]) {
+  const __amdgpu_buffer_rsrc_t rsc = gg_rsrc(C, (uint32_t)(M * N * 4));
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      const int64_t m = m0 + mb * 32 + (lane & 31);
+      const int64_t nq = n0 + nb * 32 + 4 * (lane >> 5);
+      const uint32_t base = m < M ? (uint32_t)(m + nq * M) * 4u : GG_OOB;
+      const uint32_t inc = m < M ? (uint32_t)M * 4u : 0u;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const uint32_t o = base + (uint32_t)(8 * (q >> 2) + (q & 3)) * inc;
+        const float v = acc[nb][mb][q];
+        float y = 0.f;
+        if (mode != EPI_OVERWRITE)
+          y = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsc, (int)o, 0, 0));
+        const float r = mode == EPI_OVERWRITE ? alpha * v
+                        : mode == EPI_SCALE_ONLY ? beta * y : beta * y + alpha * v;
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(r), rsc, (int)o, 0, 0);
+      }
+    }
+}
+
 template <int NQ>
 __device__ __forceinline__ void gg_stage_zero(float4 (&r)[NQ], uint32_t mask) {
   if (mask == (NQ == 8 ? 0xFFFFFFFFu : (1u << (4 * NQ)) - 1u)) return;
@@ -105,7 +181,9 @@ __device__ __forceinline__ void gg_stage_zero(float4 (&r)[NQ], uint32_t mask) {
 
 // ---- grouped GEMM, Float32 on the matrix cores -------------------------------------------
 
-template <int TM, int TN, int BK>
+// BUF: operands and C through buffer resources, A staged as rows of consecutive k (as the
+// BF16x3 kernel below does); otherwise masked loads through pointers.
+template <int TM, int TN, int BK, bool BUF>
 __global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g, const GGProb* __restrict__ probs,
                                                      const int64_t* __restrict__ first,
                                                      const int32_t* __restrict__ map) {
@@ -119,7 +197,10 @@ __global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g, const GGProb* __r
   const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
   const float alpha = (float)g.alpha.re, beta = (float)g.beta.re;
-  for (int64_t tile = blockIdx.x; tile < g.total_tiles; tile += gridDim.x) {
+  const int xc = blockIdx.x % GG_XCDS;
+  const int64_t xstep = gridDim.x / GG_XCDS;
+  for (int64_t tile = g.xcd_first[xc] + blockIdx.x / GG_XCDS; tile < g.xcd_first[xc + 1];
+       tile += xstep) {
     // the tables are restrict arguments: the kernel's stores cannot clobber them, so they are
     // read with scalar loads and every per-problem branch below is uniform
     const int p = map[tile];
@@ -139,27 +220,50 @@ __global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g, const GGProb* __r
       for (int j = 0; j < MB; ++j)
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.f;
-    // A tile (m fastest in memory): load q covers m4 = (e % (TM/4))*4, k = e / (TM/4)
+    // A tile (m fastest in memory): load q covers m4 = (e % (TM/4))*4, k = e / (TM/4);
+    // with BUF, thread t holds EPT consecutive k of row am instead
     // B tile (k fastest in memory): load q covers k4 = (e % (BK/4))*4, n = e / (BK/4)
+    constexpr int EPT = TM * BK / 256;
+    static_assert(EPT % 4 == 0 && BK % EPT == 0, "A row staging shape");
+    const int am = t % TM, ak = (t / TM) * EPT;
     float4 ra[NA], rb[NBL];
+    float rr_a[EPT];
     uint32_t mka = 0, mkb = 0;
+    __amdgpu_buffer_rsrc_t rsa, rsb;
+    if constexpr (BUF) {
+      rsa = gg_rsrc(A, (uint32_t)(M * K * 4));
+      rsb = gg_rsrc(B, (uint32_t)(K * N * 4));
+    }
+    const bool mok = m0 + am < M;
     auto load = [&](int64_t k0) __attribute__((always_inline)) {
-      mka = gg_load_a<NA>(A, M, K, pr.vec_a, m0, k0, TM, ra);
-      mkb = gg_load_b<NBL>(B, K, N, pr.vec_b, n0, k0, BK, rb);
+      if constexpr (BUF) {
+        gg_buf_load_a<EPT>(rsa, mok, m0 + am, k0 + ak, M, rr_a);
+        gg_buf_load_b<NBL, BK>(rsb, pr.vec_b, K, n0, k0, rb);
+      } else {
+        mka = gg_load_a<NA>(A, M, K, pr.vec_a, m0, k0, TM, ra);
+        mkb = gg_load_b<NBL>(B, K, N, pr.vec_b, n0, k0, BK, rb);
+      }
     };
     // an empty sum (K == 0) loads nothing: its operands may hold no element at all
     if (K > 0) load(0);
     for (int64_t k0 = 0; k0 < K; k0 += BK) {
-      gg_stage_zero<NA>(ra, mka);
-      gg_stage_zero<NBL>(rb, mkb);
+      if constexpr (BUF) {
 #pragma unroll
-      for (int q = 0; q < NA; ++q) {
-        const int e = t + 256 * q;
-        const int m = (e % (TM / 4)) * 4, k = e / (TM / 4);
-        sA[m][k] = ra[q].x;
-        sA[m + 1][k] = ra[q].y;
-        sA[m + 2][k] = ra[q].z;
-        sA[m + 3][k] = ra[q].w;
+        for (int c = 0; c < EPT / 4; ++c)
+          *(float4*)&sA[am][ak + 4 * c] =
+              make_float4(rr_a[4 * c], rr_a[4 * c + 1], rr_a[4 * c + 2], rr_a[4 * c + 3]);
+      } else {
+        gg_stage_zero<NA>(ra, mka);
+        gg_stage_zero<NBL>(rb, mkb);
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+          const int e = t + 256 * q;
+          const int m = (e % (TM / 4)) * 4, k = e / (TM / 4);
+          sA[m][k] = ra[q].x;
+          sA[m + 1][k] = ra[q].y;
+          sA[m + 2][k] = ra[q].z;
+          sA[m + 3][k] = ra[q].w;
+        }
       }
 #pragma unroll
       for (int q = 0; q < NBL; ++q) {
@@ -195,19 +299,23 @@ __global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g, const GGProb* __r
     }
     // lane holds m = m0 + wm + mb*32 + (lane & 31), n = n0 + wn + nb*32 + 8*(q/4) + 4*(lane/32) + q%4
     float* __restrict__ C = (float*)pr.c + batch * M * N;
+    if constexpr (BUF) {
+      gg_buf_store_c<NB, MB>(C, M, N, m0 + wm, n0 + wn, lane, g.mode, alpha, beta, acc);
+    } else {
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
+      for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb)
+        for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const int64_t m = m0 + wm + mb * 32 + (lane & 31);
-          const int64_t n = n0 + wn + nb * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-          if (m < M && n < N) {
-            float* yp = gg_global(C + m + n * M);
-            *yp = epilogue<OpsF32>(g.mode, alpha, beta, acc[nb][mb][q], yp);
+          for (int q = 0; q < 16; ++q) {
+            const int64_t m = m0 + wm + mb * 32 + (lane & 31);
+            const int64_t n = n0 + wn + nb * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+            if (m < M && n < N) {
+              float* yp = gg_global(C + m + n * M);
+              *yp = epilogue<OpsF32>(g.mode, alpha, beta, acc[nb][mb][q], yp);
+            }
           }
-        }
+    }
   }
 }
 
@@ -218,34 +326,46 @@ __global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g, const GGProb* __r
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ uint16_t gg_f2bf(float x) {
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+__device__ __forceinline__ uint16_t gg_f2bf(float x) {  // round to nearest even, in hardware
+  const __bf16 h = (__bf16)x;
+  uint16_t u;
+  __builtin_memcpy(&u, &h, 2);
+  return u;
 }
 __device__ __forceinline__ void gg_split(float x, uint16_t& hi, uint16_t& lo) {
   hi = gg_f2bf(x);
   lo = gg_f2bf(x - __uint_as_float((uint32_t)hi << 16));
 }
 
-// 64 x 64 tiles at five waves per SIMD (at most 102 VGPRs; 90 without spills): the many short
-// tiles are latency-bound, and the fifth resident workgroup took the 4096-problem launch from
-// 3.15 to 2.84 ms against the compiler's own choice of 106 VGPRs.  128 x 128 tiles keep the
+// 64 x 64 tiles at five waves per SIMD with buffer operands (at most 102 VGPRs, 94 used), four
+// otherwise (121 used): the many short tiles are latency-bound.  128 x 128 tiles keep the
 // compiler's choice.
-template <int TM, int TN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 ? 5 : 1, TM == 64 ? 5 : 8))) void k_grouped_bf16x3(GGDesc g, const GGProb* __restrict__ probs,
+// BUF: every problem's A, B and C batch slices span less than 2^31 bytes and M, N < 2^24 (the
+// host checks): operands are read through buffer resources with 32-bit offsets, whose range
+// check returns zero past the slice, and C is written through one, whose range check drops
+// the stores past it; only a row past M (A, C) or a k past K (B) needs an explicit check.
+// BK: k per step (32, or 64 for half the round trips of a short tile at the cost of LDS).
+template <int TM, int TN, bool BUF, int BK = 32>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 && BK == 32 ? (BUF ? 5 : 4) : 1, TM == 64 && BK == 32 ? (BUF ? 5 : 4) : 8))) void k_grouped_bf16x3(GGDesc g, const GGProb* __restrict__ probs,
                                                         const int64_t* __restrict__ first,
                                                         const int32_t* __restrict__ map) {
-  constexpr int BK = 32, LDK = 40;
+  constexpr int LDK = BK + 8;  // row pitch 80 or 144 bytes: 8 rows hit 8 distinct bank groups
   constexpr int MB = TM / 64, NB = TN / 64;
-  constexpr int NA = TM * BK / 1024, NBL = TN * BK / 1024;
+  constexpr int NBL = TN * BK / 1024;
+  // A: thread t loads EPT consecutive k of row am (lanes along m: each load is one coalesced
+  // run of the column) and stores their hi / lo halves as 16-byte rows of sA, whose 80-byte
+  // row pitch puts eight consecutive rows in distinct bank groups
+  constexpr int EPT = TM * BK / 256, KQ = BK / EPT;
+  static_assert(TM * KQ == 256 && EPT % 8 == 0 && EPT <= 32, "A staging shape");
   __shared__ __attribute__((aligned(16))) uint16_t sA[2][TM][LDK];  // [hi/lo][m][k]
   __shared__ __attribute__((aligned(16))) uint16_t sB[2][TN][LDK];  // [hi/lo][n][k]
   const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
   const float alpha = (float)g.alpha.re, beta = (float)g.beta.re;
-  for (int64_t tile = blockIdx.x; tile < g.total_tiles; tile += gridDim.x) {
+  const int xc = blockIdx.x % GG_XCDS;
+  const int64_t xstep = gridDim.x / GG_XCDS;
+  for (int64_t tile = g.xcd_first[xc] + blockIdx.x / GG_XCDS; tile < g.xcd_first[xc + 1];
+       tile += xstep) {
     // the tables are restrict arguments: the kernel's stores cannot clobber them, so they are
     // read with scalar loads and every per-problem branch below is uniform
     const int p = map[tile];
@@ -265,42 +385,60 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 ? 
       for (int j = 0; j < MB; ++j)
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[i][j][q] = 0.f;
-    float4 ra[NA], rb[NBL];
+    const int am = t % TM, ak = (t / TM) * EPT;
+    float ra[EPT];
+    float4 rb[NBL];
     uint32_t mka = 0, mkb = 0;
+    __amdgpu_buffer_rsrc_t rsa, rsb;
+    if constexpr (BUF) {
+      rsa = gg_rsrc(A, (uint32_t)(M * K * 4));
+      rsb = gg_rsrc(B, (uint32_t)(K * N * 4));
+    }
+    const bool mok = m0 + am < M;
     auto load = [&](int64_t k0) __attribute__((always_inline)) {
-      mka = gg_load_a<NA>(A, M, K, pr.vec_a, m0, k0, TM, ra);
-      mkb = gg_load_b<NBL>(B, K, N, pr.vec_b, n0, k0, BK, rb);
+      if constexpr (BUF) {
+        gg_buf_load_a<EPT>(rsa, mok, m0 + am, k0 + ak, M, ra);
+        gg_buf_load_b<NBL, BK>(rsb, pr.vec_b, K, n0, k0, rb);
+      } else {
+        // out-of-range elements read A's first element; their mask bit stays clear
+        mka = 0;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+          const int64_t k = k0 + ak + e;
+          const bool ok = mok && k < K;
+          ra[e] = gg_load(A + (ok ? m0 + am + k * M : 0));
+          mka |= ok ? 1u << e : 0u;
+        }
+        mkb = gg_load_b<NBL>(B, K, N, pr.vec_b, n0, k0, BK, rb);
+      }
     };
     // an empty sum (K == 0) loads nothing: its operands may hold no element at all
     if (K > 0) load(0);
     for (int64_t k0 = 0; k0 < K; k0 += BK) {
-      gg_stage_zero<NA>(ra, mka);
-      gg_stage_zero<NBL>(rb, mkb);
+      if constexpr (!BUF) {
+        if (mka != (EPT == 32 ? 0xFFFFFFFFu : (1u << EPT) - 1u)) {
 #pragma unroll
-      for (int q = 0; q < NA; ++q) {
-        const int e = t + 256 * q;
-        const int m = (e % (TM / 4)) * 4, k = e / (TM / 4);
-        const float v[4] = {ra[q].x, ra[q].y, ra[q].z, ra[q].w};
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          uint16_t h, l;
-          gg_split(v[i], h, l);
-          sA[0][m + i][k] = h;
-          sA[1][m + i][k] = l;
+          for (int e = 0; e < EPT; ++e) ra[e] = (mka >> e) & 1u ? ra[e] : 0.f;
         }
+        gg_stage_zero<NBL>(rb, mkb);
+      }
+#pragma unroll
+      for (int c = 0; c < EPT / 8; ++c) {
+        uint4 ph, pl;
+        split2_bf16(ra[8 * c + 0], ra[8 * c + 1], ph.x, pl.x);
+        split2_bf16(ra[8 * c + 2], ra[8 * c + 3], ph.y, pl.y);
+        split2_bf16(ra[8 * c + 4], ra[8 * c + 5], ph.z, pl.z);
+        split2_bf16(ra[8 * c + 6], ra[8 * c + 7], ph.w, pl.w);
+        *(uint4*)&sA[0][am][ak + 8 * c] = ph;
+        *(uint4*)&sA[1][am][ak + 8 * c] = pl;
       }
 #pragma unroll
       for (int q = 0; q < NBL; ++q) {
         const int e = t + 256 * q;
         const int k = (e % (BK / 4)) * 4, n = e / (BK / 4);
-        uint16_t h[4], l[4];
-        gg_split(rb[q].x, h[0], l[0]);
-        gg_split(rb[q].y, h[1], l[1]);
-        gg_split(rb[q].z, h[2], l[2]);
-        gg_split(rb[q].w, h[3], l[3]);
         uint2 ph, pl;
-        ph.x = h[0] | ((uint32_t)h[1] << 16); ph.y = h[2] | ((uint32_t)h[3] << 16);
-        pl.x = l[0] | ((uint32_t)l[1] << 16); pl.y = l[2] | ((uint32_t)l[3] << 16);
+        split2_bf16(rb[q].x, rb[q].y, ph.x, pl.x);
+        split2_bf16(rb[q].z, rb[q].w, ph.y, pl.y);
         *(uint2*)&sB[0][n][k] = ph;
         *(uint2*)&sB[1][n][k] = pl;
       }
@@ -308,7 +446,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 ? 
       if (k0 + BK < K) load(k0 + BK);
       if (m0 + wm < M && n0 + wn < N) {
 #pragma unroll
-        for (int s2 = 0; s2 < 2; ++s2) {
+        for (int s2 = 0; s2 < BK / 16; ++s2) {
           // lane l holds row l % 32, k = 16*s2 + 8*(l / 32) + 0..7
           const int kk = 16 * s2 + 8 * (lane >> 5);
           bf16x8_t ah[NB], al[NB], bh[MB], bl[MB];
@@ -335,19 +473,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 ? 
       __syncthreads();
     }
     float* __restrict__ C = (float*)pr.c + batch * M * N;
+    if constexpr (BUF) {
+      gg_buf_store_c<NB, MB>(C, M, N, m0 + wm, n0 + wn, lane, g.mode, alpha, beta, acc);
+    } else {
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
+      for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb)
+        for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const int64_t m = m0 + wm + mb * 32 + (lane & 31);
-          const int64_t n = n0 + wn + nb * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-          if (m < M && n < N) {
-            float* yp = gg_global(C + m + n * M);
-            *yp = epilogue<OpsF32>(g.mode, alpha, beta, acc[nb][mb][q], yp);
+          for (int q = 0; q < 16; ++q) {
+            const int64_t m = m0 + wm + mb * 32 + (lane & 31);
+            const int64_t n = n0 + wn + nb * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+            if (m < M && n < N) {
+              float* yp = gg_global(C + m + n * M);
+              *yp = epilogue<OpsF32>(g.mode, alpha, beta, acc[nb][mb][q], yp);
+            }
           }
-        }
+    }
   }
 }
 
@@ -361,9 +503,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 ? 
 template <int D>
 __global__ __launch_bounds__(256) void k_grouped_f32_wave(GGDesc g) {
   const int lane = threadIdx.x & 63;
-  const int64_t nw = (int64_t)gridDim.x * 4;
+  const int xc = blockIdx.x % GG_XCDS;
+  const int64_t nw = (int64_t)(gridDim.x / GG_XCDS) * 4;
   const float alpha = (float)g.alpha.re, beta = (float)g.beta.re;
-  for (int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < g.total_tiles; tile += nw) {
+  for (int64_t tile = g.xcd_first[xc] + (int64_t)(blockIdx.x / GG_XCDS) * 4 + (threadIdx.x >> 6);
+       tile < g.xcd_first[xc + 1]; tile += nw) {
     const int p = g.map[tile];
     const GGProb pr = g.probs[p];
     const int64_t r = tile - g.first[p];
@@ -447,9 +591,11 @@ __device__ __forceinline__ bf16x8w_t gg_pack8(const uint16_t (&h)[8]) {
 template <int D>
 __global__ __launch_bounds__(256) void k_grouped_bf16x3_wave(GGDesc g) {
   const int lane = threadIdx.x & 63;
-  const int64_t nw = (int64_t)gridDim.x * 4;
+  const int xc = blockIdx.x % GG_XCDS;
+  const int64_t nw = (int64_t)(gridDim.x / GG_XCDS) * 4;
   const float alpha = (float)g.alpha.re, beta = (float)g.beta.re;
-  for (int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < g.total_tiles; tile += nw) {
+  for (int64_t tile = g.xcd_first[xc] + (int64_t)(blockIdx.x / GG_XCDS) * 4 + (threadIdx.x >> 6);
+       tile < g.xcd_first[xc + 1]; tile += nw) {
     const int p = g.map[tile];
     const GGProb pr = g.probs[p];
     const int64_t r = tile - g.first[p];
@@ -541,7 +687,10 @@ __global__ __launch_bounds__(256) void k_grouped_valu(GGDesc g, const GGProb* __
   __shared__ T Bs[BK][BN + 1];
   const int t = threadIdx.x, tm = t & 15, tn = t >> 4;
   const T alpha = O::from(g.alpha), beta = O::from(g.beta);
-  for (int64_t tile = blockIdx.x; tile < g.total_tiles; tile += gridDim.x) {
+  const int xc = blockIdx.x % GG_XCDS;
+  const int64_t xstep = gridDim.x / GG_XCDS;
+  for (int64_t tile = g.xcd_first[xc] + blockIdx.x / GG_XCDS; tile < g.xcd_first[xc + 1];
+       tile += xstep) {
     const int p = map[tile];
     const GGProb pr = probs[p];
     const int64_t r = tile - first[p];
@@ -624,16 +773,30 @@ void launch_grouped_gemm(int dt, int variant, const GGDesc& g, int grid, hipStre
       hipLaunchKernelGGL((k_grouped_f32_wave<4>), dim3(grid), dim3(256), 0, st, g);
     else if (variant == GG_BF16X3_W32)
       hipLaunchKernelGGL((k_grouped_bf16x3_wave<2>), dim3(grid), dim3(256), 0, st, g);
+    else if (variant == GG_BF16X3_64_BK64 && g.offsets32)
+      hipLaunchKernelGGL((k_grouped_bf16x3<64, 64, true, 64>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
+    else if (variant == GG_BF16X3_64_BK64)
+      hipLaunchKernelGGL((k_grouped_bf16x3<64, 64, false, 64>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
+    else if (variant == GG_BF16X3_64 && g.offsets32)
+      hipLaunchKernelGGL((k_grouped_bf16x3<64, 64, true>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     else if (variant == GG_BF16X3_64)
-      hipLaunchKernelGGL((k_grouped_bf16x3<64, 64>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
+      hipLaunchKernelGGL((k_grouped_bf16x3<64, 64, false>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
+    else if (variant == GG_BF16X3_128 && g.offsets32)
+      hipLaunchKernelGGL((k_grouped_bf16x3<128, 128, true>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     else if (variant == GG_BF16X3_128)
-      hipLaunchKernelGGL((k_grouped_bf16x3<128, 128>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
+      hipLaunchKernelGGL((k_grouped_bf16x3<128, 128, false>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
+    else if (variant == GG_F32_128 && g.offsets32)
+      hipLaunchKernelGGL((k_grouped_f32<128, 128, 16, true>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     else if (variant == GG_F32_128)
-      hipLaunchKernelGGL((k_grouped_f32<128, 128, 16>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
+      hipLaunchKernelGGL((k_grouped_f32<128, 128, 16, false>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
+    else if (variant == GG_F32_64_BK32 && g.offsets32)
+      hipLaunchKernelGGL((k_grouped_f32<64, 64, 32, true>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     else if (variant == GG_F32_64_BK32)
-      hipLaunchKernelGGL((k_grouped_f32<64, 64, 32>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
+      hipLaunchKernelGGL((k_grouped_f32<64, 64, 32, false>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
+    else if (g.offsets32)
+      hipLaunchKernelGGL((k_grouped_f32<64, 64, 16, true>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     else
-      hipLaunchKernelGGL((k_grouped_f32<64, 64, 16>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
+      hipLaunchKernelGGL((k_grouped_f32<64, 64, 16, false>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     return;
   }
   OME_DISPATCH(dt, O, {

@@ -24,13 +24,6 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr int PK_K = 32;          // k per packed tile
 __host__ __device__ constexpr int pk_tile_elems(int rows) { return 2 * rows * PK_K; }
 
-__device__ __forceinline__ uint16_t f2bf_rn(float x) {
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-
 // ---- pack ----------------------------------------------------------------------------------
 // One workgroup per packed tile.  The 128x32 source tile is staged through LDS so that the
 // global reads follow whichever of (row, k) has unit stride; then every thread emits 16-byte
@@ -162,19 +155,17 @@ __device__ __forceinline__ void pack_tile(const PackDesc& d, int64_t bid, float 
   for (int q = 0; q < TR / 64; ++q) {
     const int c = t + 256 * q;
     const int r = c >> 2, kq = (c & 3) * 8;
-    uint16_t h[8], l[8];
+    float x[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int k = kq + e;
-      const float x = kmaj ? tt[k][4 * ((r >> 2) ^ pk_swz(k)) + (r & 3)] : tile[r][k];
-      h[e] = f2bf_rn(x);
-      l[e] = f2bf_rn(x - __uint_as_float((uint32_t)h[e] << 16));
+      x[e] = kmaj ? tt[k][4 * ((r >> 2) ^ pk_swz(k)) + (r & 3)] : tile[r][k];
     }
     uint4 vh, vl;
-    vh.x = h[0] | ((uint32_t)h[1] << 16); vh.y = h[2] | ((uint32_t)h[3] << 16);
-    vh.z = h[4] | ((uint32_t)h[5] << 16); vh.w = h[6] | ((uint32_t)h[7] << 16);
-    vl.x = l[0] | ((uint32_t)l[1] << 16); vl.y = l[2] | ((uint32_t)l[3] << 16);
-    vl.z = l[4] | ((uint32_t)l[5] << 16); vl.w = l[6] | ((uint32_t)l[7] << 16);
+    split2_bf16(x[0], x[1], vh.x, vl.x);
+    split2_bf16(x[2], x[3], vh.y, vl.y);
+    split2_bf16(x[4], x[5], vh.z, vl.z);
+    split2_bf16(x[6], x[7], vh.w, vl.w);
     // row r's 16-byte chunk j lives at slot j ^ ((r >> 2) & 3): conflict-free b128 reads
     const int slot = r * 4 + ((c & 3) ^ ((r >> 2) & 3));
     *(uint4*)(out + slot * 8) = vh;

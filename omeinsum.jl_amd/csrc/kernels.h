@@ -161,6 +161,11 @@ struct GGProb {  // one problem of a grouped GEMM: dense column-major, batch-maj
   int32_t tiles_m, tiles_n;
   int32_t vec_a, vec_b;  // 16-byte vector loads allowed along m (A) / k (B)
 };
+// Workgroup b runs on XCD b % GG_XCDS (the dispatcher deals workgroups to the XCDs in turn);
+// the tile list is cut into one contiguous range per XCD, each holding whole problems, so a
+// problem's operand panels are fetched into one XCD's L2.  Launch grids are multiples of
+// GG_XCDS.
+constexpr int GG_XCDS = 8;
 struct GGDesc {
   const GGProb* probs;   // device
   const int64_t* first;  // device: first tile of each problem (ascending)
@@ -169,10 +174,12 @@ struct GGDesc {
   int64_t total_tiles;
   Scal alpha, beta;
   int mode;
+  int64_t xcd_first[GG_XCDS + 1];  // tiles of XCD c: [xcd_first[c], xcd_first[c + 1])
+  int offsets32;  // every problem has M, N, K < 2^22 and batch slices of A, B, C < 2^31 bytes
 };
 enum GGVariant : int {
   GG_F32_64 = 0, GG_F32_64_BK32 = 1, GG_F32_128 = 2, GG_BF16X3_64 = 3, GG_BF16X3_128 = 4,
-  GG_F32_W32 = 5, GG_BF16X3_W32 = 6, GG_NVARIANTS = 7
+  GG_F32_W32 = 5, GG_BF16X3_W32 = 6, GG_BF16X3_64_BK64 = 7, GG_NVARIANTS = 8
 };
 void gg_tile_shape(int dt, int variant, int& tm, int& tn);
 void launch_grouped_gemm(int dt, int variant, const GGDesc& g, int grid, hipStream_t st);

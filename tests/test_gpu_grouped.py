@@ -70,14 +70,14 @@ def check(ps, got, alpha, beta, dtype, precision="fp32"):
             assert_close(g, ref, dtype, A.shape[2] + 1, precision=precision, depth=A.shape[2])
 
 
-@pytest.mark.parametrize("variant", [-1, 0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [-1, 0, 1, 2, 3, 4, 5, 6, 7])
 def test_grouped_gemm_f32_variants(variant):
     from omeinsum_mi355x import _lib
     ctx = _lib.context()
     ctx.set_option(_lib.OPT_GROUPED_VARIANT, variant)
     try:
         ps = problems(3100 + variant, 120, np.float32)
-        prec = "bf16x3" if variant in (3, 4, 6) else "fp32"  # the split tile variants
+        prec = "bf16x3" if variant in (3, 4, 6, 7) else "fp32"  # the split tile variants
         got = run(ps, True, False)
         check(ps, got, 1.0, 0.0, np.float32, prec)
         got = run(ps, 1.5, -0.5)
@@ -100,7 +100,7 @@ def test_grouped_gemm_c3_shapes_autotuned_twice(prec):
             got = run(ps, True, False)
             check(ps, got, 1.0, 0.0, np.float32, prec)
             variant = ctx.get_option(_lib.OPT_LAST_GROUPED_CHOICE) // 2
-            assert (variant in (3, 4, 6)) == (prec == "bf16x3"), variant
+            assert (variant in (3, 4, 6, 7)) == (prec == "bf16x3"), variant
     finally:
         ctx.set_option(_lib.OPT_PRECISION, 0)
 
