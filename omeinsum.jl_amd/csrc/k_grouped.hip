@@ -346,7 +346,7 @@ __device__ __forceinline__ void gg_split(float x, uint16_t& hi, uint16_t& lo) {
 // the stores past it; only a row past M (A, C) or a k past K (B) needs an explicit check.
 // BK: k per step (32, or 64 for half the round trips of a short tile at the cost of LDS).
 template <int TM, int TN, bool BUF, int BK = 32>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 && BK == 32 ? (BUF ? 5 : 4) : 1, TM == 64 && BK == 32 ? (BUF ? 5 : 4) : 8))) void k_grouped_bf16x3(GGDesc g, const GGProb* __restrict__ probs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 && BK == 32 ? (BUF ? 6 : 4) : 1, TM == 64 && BK == 32 ? (BUF ? 6 : 4) : 8))) void k_grouped_bf16x3(GGDesc g, const GGProb* __restrict__ probs,
                                                         const int64_t* __restrict__ first,
                                                         const int32_t* __restrict__ map) {
   constexpr int LDK = BK + 8;  // row pitch 80 or 144 bytes: 8 rows hit 8 distinct bank groups
