@@ -208,10 +208,8 @@ int ome_einsum(ome_ctx* ctx, int dtype, int nin, const void* const* xs, const in
  * hipMalloc).  Same arguments, semantics and errors as ome_einsum; the planner's temporaries
  * (permuted operands, split-K partials, packed BF16 operands, real-GEMM operands) come from
  * one device array of OME_NOALLOC_SCRATCH_BYTES per GPU, held by the library until the
- * process ends, instead of the pool, and the call allocates nothing (the array's first
- * 16 KiB hold the ticket counters of the one-launch split-K GEMM, zeroed at ome_init and left
- * zero by every launch; temporaries use the rest).  A contraction whose temporaries exceed
- * the rest fails with OME_ERR_OOM before anything is launched.  Every
+ * process ends, instead of the pool, and the call allocates nothing.  A contraction whose
+ * temporaries exceed the array fails with OME_ERR_OOM before anything is launched.  Every
  * call on one GPU uses that GPU's array: calls that overlap on a GPU (other contexts or
  * streams of the same device) race; calls in order on one stream do not. */
 #define OME_NOALLOC_SCRATCH_BYTES (64ll << 20)

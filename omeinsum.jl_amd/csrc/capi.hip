@@ -204,8 +204,6 @@ static __device__ __align__(256) char scratch[OME_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 constexpr int kMaxDevices = 64;
-constexpr size_t kScratchTicketBytes = SL_MAXTILES * sizeof(int32_t);
-static_assert(kScratchTicketBytes % 256 == 0, "temporaries start 256-byte aligned");
 std::mutex g_scratch_mu;
 void* g_scratch_ptr[kMaxDevices] = {};        // device address of `scratch`, per device
 hipError_t g_scratch_err[kMaxDevices] = {};  // the lookup's failure, per device
@@ -220,14 +218,6 @@ void scratch_setup(int device) {
   if (e != hipSuccess || !p) {
     hipGetLastError();
     g_scratch_err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
-    return;
-  }
-  // the head of the array holds the ticket counters of the one-launch split-K GEMM: zero
-  // here, and every launch that takes them leaves them zero
-  const hipError_t z = hipMemset(p, 0, kScratchTicketBytes);
-  if (z != hipSuccess) {
-    hipGetLastError();
-    g_scratch_err[device] = z;
     return;
   }
   g_scratch_ptr[device] = p;
@@ -289,10 +279,6 @@ int ome_shutdown(ome_ctx* ctx) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   destroy_group_state(ctx);
-  for (auto& kv : ctx->tickets) {
-    hipStreamSynchronize(kv.first);
-    hipFree(kv.second);
-  }
   delete ctx->pool;
   delete ctx;
   return OME_OK;
@@ -454,33 +440,27 @@ static int einsum_entry(ome_ctx* ctx, int dtype, int nin, const void* const* xs,
                                          hipGetErrorString(e) + ")");
   }
   // size the temporaries with a dry run of the same plan (nothing is launched)
-  // (the ticket counters at the head of the array stand in as a never-dereferenced address)
-  const size_t cap = (size_t)OME_NOALLOC_SCRATCH_BYTES - kScratchTicketBytes;
   Bump dry;
   dry.base = (char*)(uintptr_t)4096;  // never dereferenced: a dry run launches nothing
   dry.cap = (size_t)1 << 60;
   LaunchMode& lm = launch_mode();
   lm.dry = true;
   ctx->bump = &dry;
-  ctx->bump_tickets = (int32_t*)(uintptr_t)4096;
   Status r = run_einsum(ctx, c, ctx->stream);
   ctx->bump = nullptr;
-  ctx->bump_tickets = nullptr;
   lm.dry = false;
   if (!r.good()) return finish(ctx, r);
-  if (dry.high > cap)
+  if (dry.high > (size_t)OME_NOALLOC_SCRATCH_BYTES)
     return fail(ctx, OME_ERR_OOM,
                 "ome_einsum_noalloc: this contraction's temporaries need " + std::to_string(dry.high) +
-                    " bytes, above the " + std::to_string((long long)cap) +
-                    " bytes of the scratch array (use ome_einsum)");
+                    " bytes, above the " + std::to_string((long long)OME_NOALLOC_SCRATCH_BYTES) +
+                    "-byte scratch array (use ome_einsum)");
   Bump b;
-  b.base = base + kScratchTicketBytes;
-  b.cap = cap;
+  b.base = base;
+  b.cap = (size_t)OME_NOALLOC_SCRATCH_BYTES;
   ctx->bump = &b;
-  ctx->bump_tickets = (int32_t*)base;
   r = run_einsum(ctx, c, ctx->stream);
   ctx->bump = nullptr;
-  ctx->bump_tickets = nullptr;
   return finish(ctx, r);
 }
 

@@ -461,227 +461,6 @@ __global__ __launch_bounds__(PN_THREADS) void k_gemm_panel(GemmDesc d, int64_t t
   *yp = epilogue<O>(d.mode, O::from(d.sx), O::from(d.sy), W::narrow(v), yp);
 }
 
-// Few outputs, long K, in one launch with split K: workgroup (tile, s) owns a 16 x 16 output
-// tile and the s-th part of K.  Thread (u, g) holds the 4 x 4 sub-tile u (16 of them) over
-// KPT consecutive k of the part's current KC-wide chunk (g = 0..31 picks which), loading
-// them straight into registers (16-byte vectors along a contiguous m of A or k of B), so a
-// chunk costs one memory round trip and no LDS staging.  The 32 k-groups are then summed in
-// group order through LDS; the tile's partial is written, and the last of
-// the S workgroups of a tile to finish (a ticket counter per tile) adds the S partials in
-// part order, applies the epilogue and clears the ticket for the next launch on the stream.
-// The sum does not depend on which workgroup finishes last.
-constexpr int SL_THREADS = 512, SL_T = 16, SL_BATCH = 16, SL_MAXPARTS = 64;
-template <class T>
-struct SlShape {  // k per thread and per chunk: 32 operand registers per stage and thread
-  static constexpr int KPT = sizeof(T) >= 16 ? 2 : sizeof(T) >= 4 ? 4 : 8;
-  static constexpr int KC = 32 * KPT;
-};
-template <class O>
-__global__ __launch_bounds__(SL_THREADS) void k_gemm_splitk_last(GemmDesc d, int64_t tiles_m,
-                                                                 int64_t tiles_n, int64_t kchunk,
-                                                                 int S, void* partials,
-                                                                 int32_t* tickets) {
-  using T = typename O::T;
-  using W = AccOps<O>;
-  using A = typename W::A;
-  using AT = typename A::T;
-  constexpr int KPT = SlShape<T>::KPT, KC = SlShape<T>::KC;
-  // the k-groups' sums of the tile, [group][16 * (4 i + j) + u] (lanes of one store are
-  // consecutive u: no bank conflicts)
-  __shared__ AT red[SL_THREADS / 16][256];
-  __shared__ int last;
-  const int64_t tiles = tiles_m * tiles_n * d.B;
-  const int64_t tile = blockIdx.x % tiles;
-  const int s = (int)(blockIdx.x / tiles);
-  int64_t r = tile;
-  const int64_t tm = r % tiles_m;
-  r /= tiles_m;
-  const int64_t tn = r % tiles_n;
-  const int64_t batch = r / tiles_n;
-  const int64_t m0 = tm * SL_T, n0 = tn * SL_T;
-  const T* __restrict__ ap = (const T*)d.a + batch * d.sab;
-  const T* __restrict__ bp = (const T*)d.b + batch * d.sbb;
-  const int64_t k0 = s * kchunk;
-  const int64_t k1 = k0 + kchunk < d.K ? k0 + kchunk : d.K;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const int u = lane & 15, g = (lane >> 4) + 4 * wv;  // sub-tile, k-group
-  const int64_t mu = m0 + 4 * (u & 3), nu = n0 + 4 * (u >> 2);
-  // 16-byte vectors: 4 rows of A at one k (m contiguous), KPT k of B at one n (k contiguous)
-  const bool va = d.sam == 1 && mu + 4 <= d.M && (sizeof(T) * 4) % 16 == 0 &&
-                  (((uintptr_t)(ap + mu)) % 16) == 0 && (d.sak * sizeof(T)) % 16 == 0;
-  const bool vb = d.sbk == 1 && nu + 4 <= d.N && (sizeof(T) * KPT) % 16 == 0 &&
-                  (((uintptr_t)bp) % 16) == 0 && (d.sbn * sizeof(T)) % 16 == 0 && (k0 % KPT) == 0;
-  AT acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = A::zero();
-  // operands of chunk kb into registers: one round trip
-  auto load = [&](int64_t kb, T (&av)[KPT][4], T (&bv)[4][KPT]) __attribute__((always_inline)) {
-    const int64_t kt = kb + (int64_t)g * KPT;  // this thread's first k
-    const bool kfull = kt + KPT <= k1;
-    if (va && kfull) {
-#pragma unroll
-      for (int q = 0; q < KPT; ++q)
-        __builtin_memcpy(av[q], __builtin_assume_aligned(ap + mu + (kt + q) * d.sak, 16),
-                         sizeof(av[q]));
-    } else {
-      const T* pa = ap + kt * d.sak;
-#pragma unroll
-      for (int q = 0; q < KPT; ++q, pa += d.sak)
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          av[q][i] = (mu + i < d.M && kt + q < k1) ? pa[(mu + i) * d.sam] : O::zero();
-    }
-    if (vb && kfull) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        __builtin_memcpy(bv[j], __builtin_assume_aligned(bp + kt + (nu + j) * d.sbn, 16),
-                         sizeof(bv[j]));
-    } else {
-      const T* pb = bp + kt * d.sbk;
-#pragma unroll
-      for (int q = 0; q < KPT; ++q, pb += d.sbk)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          bv[j][q] = (nu + j < d.N && kt + q < k1) ? pb[(nu + j) * d.sbn] : O::zero();
-    }
-  };
-  auto fma_chunk = [&](const T (&av)[KPT][4], const T (&bv)[4][KPT]) __attribute__((always_inline)) {
-#pragma unroll
-    for (int q = 0; q < KPT; ++q)
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = A::fma(W::widen(av[q][i]), W::widen(bv[j][q]), acc[i][j]);
-  };
-  // two register stages: the loads of the next chunk are in flight during this chunk's products
-  T a0[KPT][4], b0[4][KPT], a1[KPT][4], b1[4][KPT];
-  int64_t kb = k0;
-  if (kb < k1) load(kb, a0, b0);
-  while (kb < k1) {
-    const int64_t kn = kb + KC;
-    if (kn < k1) load(kn, a1, b1);
-    fma_chunk(a0, b0);
-    if (kn >= k1) break;
-    const int64_t kn2 = kn + KC;
-    if (kn2 < k1) load(kn2, a0, b0);
-    fma_chunk(a1, b1);
-    kb = kn2;
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) red[g][16 * (4 * i + j) + u] = acc[i][j];
-  __syncthreads();
-  // threads 0..255 own output o = 16 (4 i + j) + u (row mu + i, column nu + j of sub-tile u)
-  const int o = t & 255, h = t >> 8;
-  AT v0 = red[0][o];
-#pragma unroll 8
-  for (int q = 1; q < SL_THREADS / 16; ++q) v0 = A::add(v0, red[q][o]);
-  const int ou = o & 15, oi = o >> 6, oj = (o >> 4) & 3;
-  const int64_t om = m0 + 4 * (ou & 3) + oi, on = n0 + 4 * (ou >> 2) + oj;
-  const bool out = h == 0 && om < d.M && on < d.N;
-  T* yp = (T*)d.y + batch * d.syb + om * d.sym + on * d.syn;
-  if (S == 1) {
-    if (out) *yp = epilogue<O>(d.mode, O::from(d.sx), O::from(d.sy), W::narrow(v0), yp);
-    return;
-  }
-  AT acc0 = v0;
-  // Partials, tickets and the partial reads are agent-scope atomics (coherent across the
-  // XCDs' L2s on their own), ordered by waiting for the stores' completion before the ticket
-  // is taken; a device-wide fence would write back and invalidate the whole L2 instead.
-  constexpr int PW = (int)((sizeof(AT) + 3) / 4);  // 32-bit words per partial
-  uint32_t* P = (uint32_t*)partials;
-  if (h == 0) {
-    uint32_t w[PW] = {};
-    __builtin_memcpy(w, &acc0, sizeof(AT));
-    uint32_t* dst = P + (((int64_t)s * tiles + tile) * 256 + o) * PW;
-#pragma unroll
-    for (int i = 0; i < PW; ++i)
-      __hip_atomic_store(dst + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __builtin_amdgcn_s_waitcnt(0);  // every store of this thread is performed
-  __syncthreads();
-  if (t == 0)
-    last = __hip_atomic_fetch_add(&tickets[tile], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-           S - 1;
-  __syncthreads();
-  if (!last) return;  // uniform over the workgroup
-  // thread (o, h) sums parts [h * S2, min(S, (h + 1) * S2)) in order, loading SL_BATCH of
-  // them before adding them; output o is then part sum 0 + part sum 1
-  const int S2 = (S + 1) / 2;
-  const int qe = (h + 1) * S2 < S ? (h + 1) * S2 : S;
-  AT v = A::zero();
-  bool any = false;
-  for (int q0 = h * S2; q0 < qe; q0 += SL_BATCH) {
-    uint32_t w[SL_BATCH][PW];
-#pragma unroll
-    for (int b = 0; b < SL_BATCH; ++b) {
-      if (q0 + b < qe) {
-        const uint32_t* src = P + (((int64_t)(q0 + b) * tiles + tile) * 256 + o) * PW;
-#pragma unroll
-        for (int i = 0; i < PW; ++i)
-          w[b][i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < SL_BATCH; ++b) {
-      if (q0 + b < qe) {
-        AT x;
-        __builtin_memcpy(&x, w[b], sizeof(AT));
-        v = any ? A::add(v, x) : x;
-        any = true;
-      }
-    }
-  }
-  if (h == 1) red[0][o] = v;
-  __syncthreads();
-  if (h == 0) {
-    if (qe < S) v = A::add(v, red[0][o]);  // the second half holds parts
-    if (out) *yp = epilogue<O>(d.mode, O::from(d.sx), O::from(d.sy), W::narrow(v), yp);
-  }
-  if (t == 0) __hip_atomic_store(&tickets[tile], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-bool splitk_last_fits(const GemmDesc& d) {
-  return d.M >= SL_T && d.N >= SL_T &&
-         ((d.M + SL_T - 1) / SL_T) * ((d.N + SL_T - 1) / SL_T) * d.B <= SL_MAXTILES;
-}
-
-void splitk_last_plan(int dt, const GemmDesc& d, int& S, int64_t& kchunk, int64_t& ws_bytes) {
-  const int64_t tiles = ((d.M + SL_T - 1) / SL_T) * ((d.N + SL_T - 1) / SL_T) * d.B;
-  const int64_t es = elem_size(dt);
-  const int64_t kc = 32 * (es >= 16 ? 2 : es >= 4 ? 4 : 8);  // SlShape<T>::KC
-  const int64_t chunks = (d.K + kc - 1) / kc;
-  int64_t s = 256 / tiles;  // one workgroup per CU (each holds up to 160 KiB of LDS)
-  if (s > SL_MAXPARTS) s = SL_MAXPARTS;
-  if (s > chunks) s = chunks;
-  if (s < 1) s = 1;
-  kchunk = ((chunks + s - 1) / s) * kc;
-  S = (int)((d.K + kchunk - 1) / kchunk);
-  // partial sums in the accumulation type (twice the element size for Float32 / ComplexF32),
-  // in whole 32-bit words
-  int64_t as = (dt == F32 || dt == C32) ? 2 * es : es;
-  as = (as + 3) / 4 * 4;
-  ws_bytes = S > 1 ? (int64_t)S * tiles * 256 * as : 0;
-}
-
-void launch_gemm_splitk_last(int dt, GemmDesc d, int S, int64_t kchunk, void* partials,
-                             int32_t* tickets, hipStream_t st) {
-  OME_DRY_RETURN(1);
-  const int64_t tiles_m = (d.M + SL_T - 1) / SL_T, tiles_n = (d.N + SL_T - 1) / SL_T;
-  const int64_t blocks = tiles_m * tiles_n * d.B * S;
-  d.ksplit = 1;
-  OME_DISPATCH(dt, O, {
-    d.mode = epi_mode<O>(d.sx, d.sy);
-    TimedScope ts(KC_GEMM_VALU, st);
-    hipLaunchKernelGGL(k_gemm_splitk_last<O>, dim3((uint32_t)blocks), dim3(SL_THREADS), 0, st, d,
-                       tiles_m, tiles_n, kchunk, S, partials, tickets);
-  });
-}
-
 bool panel_fits(const GemmDesc& d) {
   const int64_t outputs = d.M * d.N * d.B;
   return outputs > 1024 && outputs <= 16384 && d.K >= 512 && outputs * d.K <= (1ll << 26);

@@ -123,14 +123,6 @@ void launch_fused_jobs(int dt, const FusedJob* jobs, int njobs, int nsub_total, 
 // workgroup per 4x4 (or 1x16, 16x1) output tile over all of K, reduced in a fixed order.
 bool panel_fits(const GemmDesc& d);
 void launch_gemm_panel(int dt, GemmDesc d, hipStream_t st);
-// The same problems with M, N >= 16 split over K in one launch: 16 x 16 tiles, S parts each;
-// the last part of a tile to finish sums the S partials in order.  tickets: SL_MAXTILES
-// zeroed counters owned by the stream (each launch leaves them zero); partials: ws_bytes.
-constexpr int64_t SL_MAXTILES = 4096;
-bool splitk_last_fits(const GemmDesc& d);
-void splitk_last_plan(int dt, const GemmDesc& d, int& S, int64_t& kchunk, int64_t& ws_bytes);
-void launch_gemm_splitk_last(int dt, GemmDesc d, int S, int64_t kchunk, void* partials,
-                             int32_t* tickets, hipStream_t st);
 
 // Few outputs (M*N*batch <= 1024), long K: one workgroup per output and K part.
 int dot_plan_parts(const GemmDesc& d);

@@ -288,54 +288,12 @@ static Status run_gemm_via_real(ome_ctx* ctx, int dt, const GemmDesc& d, hipStre
   return Status::ok();
 }
 
-static bool stream_capturing(hipStream_t st) {
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
-    hipGetLastError();
-    return true;
-  }
-  return cs != hipStreamCaptureStatusNone;
-}
-
-// The stream's ticket counters, allocated and zeroed in stream order on its first use
-// (outside capture).
-static int32_t* stream_tickets(ome_ctx* ctx, hipStream_t st) {
-  auto it = ctx->tickets.find(st);
-  if (it != ctx->tickets.end()) return it->second;
-  int32_t* p = nullptr;
-  if (hipMalloc(&p, SL_MAXTILES * sizeof(int32_t)) != hipSuccess) {
-    hipGetLastError();
-    return nullptr;
-  }
-  if (hipMemsetAsync(p, 0, SL_MAXTILES * sizeof(int32_t), st) != hipSuccess) {
-    hipGetLastError();
-    hipFree(p);
-    return nullptr;
-  }
-  ctx->tickets[st] = p;
-  return p;
-}
-
 Status run_gemm(ome_ctx* ctx, int dt, GemmDesc d, hipStream_t st, int precision) {
   if (d.M <= 0 || d.N <= 0 || d.B <= 0) return Status::ok();
   int kernel = ctx->opt.gemm_kernel;
   // few outputs over a long K (up to 2^26 products): one launch, latency-bound, summed in the
-  // accumulation arithmetic (FP64 for Float32), so at least as precise as every path below.
-  // Split over K with a last-workgroup reduction where ticket counters are at hand (the
-  // stream's own, or the scratch array's under ome_einsum_noalloc) and the stream is not
-  // being captured; otherwise one workgroup per small tile over all of K.
+  // accumulation arithmetic (FP64 for Float32), so at least as precise as every path below
   if (kernel == GEMM_AUTO && panel_fits(d)) {
-    if (splitk_last_fits(d) && !stream_capturing(st)) {
-      int32_t* tk = ctx->bump ? ctx->bump_tickets : stream_tickets(ctx, st);
-      int S = 1;
-      int64_t kchunk = 0, wsb = 0;
-      splitk_last_plan(dt, d, S, kchunk, wsb);
-      Temp ws(ctx, (size_t)wsb);
-      if (tk && (wsb == 0 || ws.p)) {
-        launch_gemm_splitk_last(dt, d, S, kchunk, ws.p, tk, st);
-        return Status::ok();
-      }
-    }
     launch_gemm_panel(dt, d, st);
     return Status::ok();
   }

@@ -98,12 +98,6 @@ struct ome_ctx {
   ome::Pool* pool = nullptr;
   ome::Options opt;
   ome_device_info info{};
-  // per stream: the zeroed ticket counters of the one-launch split-K GEMM (launches on one
-  // stream run in order, so they share the counters; each launch leaves them zero)
-  std::unordered_map<hipStream_t, int32_t*> tickets;
-  // the counters to use while `bump` is set: the head of ome_einsum_noalloc's scratch array
-  // (a never-dereferenced stand-in during its sizing dry run); null for other bumps
-  int32_t* bump_tickets = nullptr;
 };
 
 namespace ome {

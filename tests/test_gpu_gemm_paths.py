@@ -262,11 +262,10 @@ def test_c4_shaped_gradient_finite_differences():
     assert abs(fb - np.sum(gb * vb)) <= 1e-7 * max(1.0, abs(fb))
 
 
-def test_one_launch_split_k_is_deterministic_across_calls():
-    """Few outputs / long K with M, N >= 16 run split over K in one launch, the last
-    workgroup of each tile summing the partials in split order: repeated calls, interleaved
-    with calls of other tile counts that share the stream's ticket counters, give bit-equal
-    results, within the FP32 bound of the FP64 oracle."""
+def test_one_launch_few_outputs_is_deterministic_across_calls():
+    """Few outputs / long K run in one launch, each workgroup summing its tile over all of K
+    and reducing its threads' sums in a fixed tree: repeated calls, interleaved with calls of
+    other shapes, give bit-equal results, within the FP32 bound of the FP64 oracle."""
     o = om()
     cases = [("ijk,jkl->il", {"i": 64, "j": 64, "k": 64, "l": 64}),
              ("ijk,jkl->il", {"i": 30, "j": 20, "k": 30, "l": 37}),
