@@ -461,7 +461,9 @@ __global__ __launch_bounds__(PN_THREADS) void k_gemm_panel(GemmDesc d, int64_t t
   *yp = epilogue<O>(d.mode, O::from(d.sx), O::from(d.sy), W::narrow(v), yp);
 }
 
-bool panel_fits(const GemmDesc& d) {
+bool panel_fits(int dt, const GemmDesc& d) {
+  // complex products keep their matrix-core routes (the realified kernels)
+  if (dt == C32 || dt == C64) return false;
   const int64_t outputs = d.M * d.N * d.B;
   return outputs > 1024 && outputs <= 16384 && d.K >= 512 && outputs * d.K <= (1ll << 26);
 }

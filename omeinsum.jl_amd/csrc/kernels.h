@@ -119,9 +119,10 @@ struct FusedJob {
 constexpr int SF_MAXJOBS = 64;
 void launch_fused_jobs(int dt, const FusedJob* jobs, int njobs, int nsub_total, hipStream_t st);
 
-// Few outputs (1024 < M*N*batch <= 16384), K >= 512, at most 2^26 products: one launch, a
-// workgroup per 4x4 (or 1x16, 16x1) output tile over all of K, reduced in a fixed order.
-bool panel_fits(const GemmDesc& d);
+// Few outputs (1024 < M*N*batch <= 16384), K >= 512, at most 2^26 products, not complex: one
+// launch, a workgroup per 4x4 (or 1x16, 16x1) output tile over all of K, reduced in a fixed
+// order.
+bool panel_fits(int dt, const GemmDesc& d);
 void launch_gemm_panel(int dt, GemmDesc d, hipStream_t st);
 
 // Few outputs (M*N*batch <= 1024), long K: one workgroup per output and K part.

@@ -293,7 +293,7 @@ Status run_gemm(ome_ctx* ctx, int dt, GemmDesc d, hipStream_t st, int precision)
   int kernel = ctx->opt.gemm_kernel;
   // few outputs over a long K (up to 2^26 products): one launch, latency-bound, summed in the
   // accumulation arithmetic (FP64 for Float32), so at least as precise as every path below
-  if (kernel == GEMM_AUTO && panel_fits(d)) {
+  if (kernel == GEMM_AUTO && panel_fits(dt, d)) {
     launch_gemm_panel(dt, d, st);
     return Status::ok();
   }
