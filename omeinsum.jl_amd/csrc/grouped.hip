@@ -215,6 +215,8 @@ static GroupEntry* gg_entry(ome_ctx* ctx, hipStream_t st, int dt, int variant,
       g.tiles_n = (int32_t)((p.n + tn - 1) / tn);
       g.vec_a = (p.m % 4 == 0 && aligned16(p.a)) ? 1 : 0;
       g.vec_b = (p.k % 4 == 0 && aligned16(p.b)) ? 1 : 0;
+      g.per_div = make_fdiv((uint32_t)((int64_t)g.tiles_m * g.tiles_n));
+      g.tm_div = make_fdiv((uint32_t)g.tiles_m);
       probs[np] = g;
       first[np] = total;
       const int64_t nt = (int64_t)g.tiles_m * g.tiles_n * p.batch;
@@ -227,8 +229,9 @@ static GroupEntry* gg_entry(ome_ctx* ctx, hipStream_t st, int dt, int variant,
     for (const auto& p : ps) {
       if (p.m <= 0 || p.n <= 0 || p.batch <= 0) continue;
       const int64_t lim = (int64_t)1 << 22, slim = (int64_t)1 << 31;
+      const int64_t ptiles = ((p.m + tm - 1) / tm) * ((p.n + tn - 1) / tn) * p.batch;
       if (p.m >= lim || p.n >= lim || p.k >= lim || p.m * p.k * 4 >= slim ||
-          p.k * p.n * 4 >= slim || p.m * p.n * 4 >= slim)
+          p.k * p.n * 4 >= slim || p.m * p.n * 4 >= slim || ptiles >= slim)
         e.offsets32 = false;
     }
     e.first_off = pbytes;

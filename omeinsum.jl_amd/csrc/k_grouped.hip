@@ -205,11 +205,22 @@ __global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g, const GGProb* __r
     // read with scalar loads and every per-problem branch below is uniform
     const int p = map[tile];
     const GGProb pr = probs[p];
-    const int64_t r = tile - first[p];
-    const int64_t per = (int64_t)pr.tiles_m * pr.tiles_n;
-    const int64_t batch = r / per;
-    const int64_t rr = r - batch * per;
-    const int64_t m0 = (rr % pr.tiles_m) * TM, n0 = (rr / pr.tiles_m) * TN;
+    int64_t batch, m0, n0;
+    if constexpr (BUF) {  // the problem's tile count fits 32 bits: invariant division
+      const uint32_t r = (uint32_t)(tile - first[p]);
+      const uint32_t b = fdiv_q(r, pr.per_div), rr = r - b * pr.per_div.d;
+      const uint32_t tn = fdiv_q(rr, pr.tm_div);
+      batch = b;
+      m0 = (int64_t)(rr - tn * pr.tm_div.d) * TM;
+      n0 = (int64_t)tn * TN;
+    } else {
+      const int64_t r = tile - first[p];
+      const int64_t per = (int64_t)pr.tiles_m * pr.tiles_n;
+      batch = r / per;
+      const int64_t rr = r - batch * per;
+      m0 = (rr % pr.tiles_m) * TM;
+      n0 = (rr / pr.tiles_m) * TN;
+    }
     const int64_t M = pr.m, N = pr.n, K = pr.k;
     const float* __restrict__ A = (const float*)pr.a + batch * M * K;
     const float* __restrict__ B = (const float*)pr.b + batch * K * N;
@@ -370,11 +381,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 &&
     // read with scalar loads and every per-problem branch below is uniform
     const int p = map[tile];
     const GGProb pr = probs[p];
-    const int64_t r = tile - first[p];
-    const int64_t per = (int64_t)pr.tiles_m * pr.tiles_n;
-    const int64_t batch = r / per;
-    const int64_t rr = r - batch * per;
-    const int64_t m0 = (rr % pr.tiles_m) * TM, n0 = (rr / pr.tiles_m) * TN;
+    int64_t batch, m0, n0;
+    if constexpr (BUF) {  // the problem's tile count fits 32 bits: invariant division
+      const uint32_t r = (uint32_t)(tile - first[p]);
+      const uint32_t b = fdiv_q(r, pr.per_div), rr = r - b * pr.per_div.d;
+      const uint32_t tn = fdiv_q(rr, pr.tm_div);
+      batch = b;
+      m0 = (int64_t)(rr - tn * pr.tm_div.d) * TM;
+      n0 = (int64_t)tn * TN;
+    } else {
+      const int64_t r = tile - first[p];
+      const int64_t per = (int64_t)pr.tiles_m * pr.tiles_n;
+      batch = r / per;
+      const int64_t rr = r - batch * per;
+      m0 = (rr % pr.tiles_m) * TM;
+      n0 = (rr / pr.tiles_m) * TN;
+    }
     const int64_t M = pr.m, N = pr.n, K = pr.k;
     const float* __restrict__ A = (const float*)pr.a + batch * M * K;
     const float* __restrict__ B = (const float*)pr.b + batch * K * N;

@@ -153,6 +153,7 @@ struct GGProb {  // one problem of a grouped GEMM: dense column-major, batch-maj
   int64_t m, n, k;
   int32_t tiles_m, tiles_n;
   int32_t vec_a, vec_b;  // 16-byte vector loads allowed along m (A) / k (B)
+  FDiv per_div, tm_div;  // invariant division by tiles_m * tiles_n and by tiles_m
 };
 // Workgroup b runs on XCD b % GG_XCDS (the dispatcher deals workgroups to the XCDs in turn);
 // the tile list is cut into one contiguous range per XCD, each holding whole problems, so a
