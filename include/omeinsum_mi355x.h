@@ -102,9 +102,12 @@ typedef enum ome_option {
                                   step 64 (Float32 only; for tests) */
   OME_OPT_LAST_GROUPED_CHOICE = 9, /* read-only: 2*variant + persistent of the last grouped
                                       GEMM launch (the autotuner's pick) */
-  OME_OPT_TREE_DEFER_MEMORY = 10   /* 1: ome_tree_create sizes the plan's device memory but
+  OME_OPT_TREE_DEFER_MEMORY = 10,  /* 1: ome_tree_create sizes the plan's device memory but
                                       does not allocate it (first use or the request loop
                                       does); ome_tree_get_info reports the size */
+  OME_OPT_GROUPED_BUFFERS = 11     /* 1 (default): Float32 grouped GEMMs whose problems all fit
+                                      32-bit offsets read and write through buffer resources;
+                                      0: always through pointers (for tests) */
 } ome_option;
 
 /* Kernel classes of ome_get_kernel_timing. */

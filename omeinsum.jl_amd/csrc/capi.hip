@@ -310,6 +310,7 @@ int ome_set_option(ome_ctx* ctx, int key, int64_t v) {
       ctx->opt.gemm_kernel = (int)v; break;
     case OME_OPT_KERNEL_TIMING: timing_enable(v != 0); break;
     case OME_OPT_TREE_DEFER_MEMORY: ctx->opt.defer_tree_mem = (int)(v != 0); break;
+    case OME_OPT_GROUPED_BUFFERS: ctx->opt.grouped_buffers = (int)(v != 0); break;
     case OME_OPT_GROUPED_VARIANT:
       if (v < -1 || v >= GG_NVARIANTS) return fail(ctx, OME_ERR_INVALID_ARG, "grouped variant must be -1..7");
       ctx->opt.grouped_variant = (int)v; break;
@@ -333,6 +334,7 @@ int ome_get_option(ome_ctx* ctx, int key, int64_t* v) {
     case OME_OPT_GROUPED_VARIANT: *v = ctx->opt.grouped_variant; break;
     case OME_OPT_LAST_GROUPED_CHOICE: *v = ctx->last_grouped_choice; break;
     case OME_OPT_TREE_DEFER_MEMORY: *v = ctx->opt.defer_tree_mem; break;
+    case OME_OPT_GROUPED_BUFFERS: *v = ctx->opt.grouped_buffers; break;
     default: return fail(ctx, OME_ERR_INVALID_ARG, "unknown option");
   }
   return OME_OK;

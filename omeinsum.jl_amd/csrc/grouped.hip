@@ -305,7 +305,7 @@ static Status gg_launch(ome_ctx* ctx, hipStream_t st, int dt, const GGCand& c,
   g.beta = beta;
   g.mode = dt == F32 ? epi_mode<OpsF32>(alpha, beta) : 0;
   for (int x = 0; x <= GG_XCDS; ++x) g.xcd_first[x] = e->xfirst[x];
-  g.offsets32 = e->offsets32 ? 1 : 0;
+  g.offsets32 = e->offsets32 && ctx->opt.grouped_buffers ? 1 : 0;
   launch_grouped_gemm(dt, c.variant, g, gg_grid(ctx, dt, c, *e), st);
   return Status::ok();
 }
@@ -474,6 +474,7 @@ static int64_t gp_plan(int dt, const std::vector<int64_t>& od, const std::vector
     pr.tlb_div = make_fdiv((uint32_t)s_pb);
     pr.tsc = (int32_t)s_tsc;
     pr.tsc_div = make_fdiv((uint32_t)s_tsc);
+    pr.st_div = make_fdiv((uint32_t)(s_tsc * s_pb));
     pr.s_ext = od[sq];
     pr.s_is = os[sq];
     pr.s_os = ost[sq];

@@ -892,19 +892,21 @@ __global__ __launch_bounds__(256) void k_grouped_permute(GPDesc g) {
       const uint32_t cnt = (uint32_t)tla * st;                     // <= SCAP
       const T* __restrict__ xb = x + ibase;
       T* __restrict__ yb = y + obase;
-      // read: (b, s) runs contiguous in x across lanes, a slowest
+      // read: (b, s) runs contiguous in x across lanes, a slowest.  S follows I in the input
+      // (s_is == tlb), so element e = (a, r) with r = s * tlb + b over the run of one a sits
+      // at x offset r + fa(a), and at LDS index a * lda + r: one division per element
       constexpr int NBT = sizeof(T) >= 16 ? 8 : 16;
+      const uint32_t rvalid = nsv * (uint32_t)tlb;  // r past the chunk's last s is not read
       for (uint32_t h = 0; h < cnt; h += 256u * NBT) {
         T v[NBT];
         uint32_t li[NBT];
 #pragma unroll
         for (int j = 0; j < NBT; ++j) {
           const uint32_t e = h + threadIdx.x + 256u * j;
-          const uint32_t t1 = fdiv_q(e, pr.tlb_div), bl = e - t1 * (uint32_t)tlb;
-          const uint32_t al = fdiv_q(t1, pr.tsc_div), sl = t1 - al * (uint32_t)tsc;
-          li[j] = (e < cnt) ? al * lda + sl * (uint32_t)tlb + bl : 0xFFFFFFFFu;
+          const uint32_t al = fdiv_q(e, pr.st_div), r = e - al * st;
+          li[j] = (e < cnt) ? al * lda + r : 0xFFFFFFFFu;
           v[j] = O::zero();
-          if (e < cnt && sl < nsv) v[j] = xb[bl + (int64_t)sl * pr.s_is + soff_a[al]];
+          if (e < cnt && r < rvalid) v[j] = xb[r + soff_a[al]];
         }
 #pragma unroll
         for (int j = 0; j < NBT; ++j)

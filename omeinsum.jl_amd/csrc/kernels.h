@@ -209,6 +209,7 @@ struct GPProb {
   // that follows I in the input and O in the output (runs of pb * tsc and pa * tsc elements)
   int32_t tsc;
   FDiv tsc_div;
+  FDiv st_div;  // tiled == 2: division by tsc * pb, the input run of one a
   int64_t s_ext, s_is, s_os, ts_tiles;
 };
 struct GPDesc {

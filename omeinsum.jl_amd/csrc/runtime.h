@@ -44,6 +44,7 @@ struct Options {
   int gemm_kernel = GEMM_AUTO;
   int grouped_variant = -1;  // -1: autotuned; else the grouped GEMM tile variant
   int defer_tree_mem = 0;    // 1: tree plans get device memory on first use, not at creation
+  int grouped_buffers = 1;   // 1: grouped GEMMs may use the buffer-resource kernels
 };
 
 struct Status {

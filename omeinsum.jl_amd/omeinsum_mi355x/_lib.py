@@ -16,6 +16,7 @@ F32, F64, C32, C64, I64, TF32, TF64, DD, CT64, TBOOL = range(10)
 OPT_PRECISION, OPT_COMPLEX_ALGO, OPT_ALLOW_LOOPS, OPT_DIRECT_GEMM = 0, 1, 2, 3
 OPT_USE_GRAPHS, OPT_AUTOTUNE, OPT_GEMM_KERNEL, OPT_KERNEL_TIMING = 4, 5, 6, 7
 OPT_GROUPED_VARIANT, OPT_LAST_GROUPED_CHOICE, OPT_TREE_DEFER_MEMORY = 8, 9, 10
+OPT_GROUPED_BUFFERS = 11
 # the options a tree plan freezes at creation
 PLAN_OPTIONS = (OPT_PRECISION, OPT_COMPLEX_ALGO, OPT_ALLOW_LOOPS, OPT_DIRECT_GEMM, OPT_GEMM_KERNEL)
 # kernel classes of ome_get_kernel_timing

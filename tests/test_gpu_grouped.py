@@ -70,11 +70,15 @@ def check(ps, got, alpha, beta, dtype, precision="fp32"):
             assert_close(g, ref, dtype, A.shape[2] + 1, precision=precision, depth=A.shape[2])
 
 
+@pytest.mark.parametrize("buffers", [1, 0])
 @pytest.mark.parametrize("variant", [-1, 0, 1, 2, 3, 4, 5, 6, 7])
-def test_grouped_gemm_f32_variants(variant):
+def test_grouped_gemm_f32_variants(variant, buffers):
+    # buffers 0 forces the pointer form of the kernels that otherwise read through buffer
+    # resources (the form problems above 32-bit offsets take)
     from omeinsum_mi355x import _lib
     ctx = _lib.context()
     ctx.set_option(_lib.OPT_GROUPED_VARIANT, variant)
+    ctx.set_option(_lib.OPT_GROUPED_BUFFERS, buffers)
     try:
         ps = problems(3100 + variant, 120, np.float32)
         prec = "bf16x3" if variant in (3, 4, 6, 7) else "fp32"  # the split tile variants
@@ -84,6 +88,7 @@ def test_grouped_gemm_f32_variants(variant):
         check(ps, got, 1.5, -0.5, np.float32, prec)
     finally:
         ctx.set_option(_lib.OPT_GROUPED_VARIANT, -1)
+        ctx.set_option(_lib.OPT_GROUPED_BUFFERS, 1)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
