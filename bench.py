@@ -343,7 +343,7 @@ def bench_c3(steps, warmup, per_problem_path=True):
            "permute_bytes": perm_bytes}
     names = {0: "f32 64x64", 1: "f32 64x64 bk32", 2: "f32 128x128", 3: "bf16x3 64x64",
              4: "bf16x3 128x128", 5: "f32 wave 32x32", 6: "bf16x3 wave 32x32",
-             7: "bf16x3 64x64 bk64"}
+             7: "bf16x3 64x64 bk64", 8: "f32 64x64 runs", 9: "bf16x3 64x64 runs"}
     for prec in ("fp32", "bf16x3"):
         ctx.set_option(_lib.OPT_PRECISION, _lib.PREC[prec])
         gg()  # first call: autotune the tile variant and upload the descriptors

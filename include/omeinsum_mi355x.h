@@ -99,7 +99,10 @@ typedef enum ome_option {
                                   1 64x64 with K step 32, 2 128x128, 3 BF16x3 64x64,
                                   4 BF16x3 128x128, 5 one wave per 32x32 tile,
                                   6 BF16x3 one wave per 32x32 tile, 7 BF16x3 64x64 with K
-                                  step 64 (Float32 only; for tests) */
+                                  step 64, 8 64x64 persistent runs of consecutive tiles,
+                                  9 BF16x3 64x64 persistent runs (8 and 9 need buffer
+                                  resources and otherwise run as 1 and 3; Float32 only; for
+                                  tests) */
   OME_OPT_LAST_GROUPED_CHOICE = 9, /* read-only: 2*variant + persistent of the last grouped
                                       GEMM launch (the autotuner's pick) */
   OME_OPT_TREE_DEFER_MEMORY = 10,  /* 1: ome_tree_create sizes the plan's device memory but

@@ -312,7 +312,7 @@ int ome_set_option(ome_ctx* ctx, int key, int64_t v) {
     case OME_OPT_TREE_DEFER_MEMORY: ctx->opt.defer_tree_mem = (int)(v != 0); break;
     case OME_OPT_GROUPED_BUFFERS: ctx->opt.grouped_buffers = (int)(v != 0); break;
     case OME_OPT_GROUPED_VARIANT:
-      if (v < -1 || v >= GG_NVARIANTS) return fail(ctx, OME_ERR_INVALID_ARG, "grouped variant must be -1..7");
+      if (v < -1 || v >= GG_NVARIANTS) return fail(ctx, OME_ERR_INVALID_ARG, "grouped variant must be -1..9");
       ctx->opt.grouped_variant = (int)v; break;
     default: return fail(ctx, OME_ERR_INVALID_ARG, "unknown option");
   }

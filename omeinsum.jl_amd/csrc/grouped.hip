@@ -217,6 +217,7 @@ static GroupEntry* gg_entry(ome_ctx* ctx, hipStream_t st, int dt, int variant,
       g.vec_b = (p.k % 4 == 0 && aligned16(p.b)) ? 1 : 0;
       g.per_div = make_fdiv((uint32_t)((int64_t)g.tiles_m * g.tiles_n));
       g.tm_div = make_fdiv((uint32_t)g.tiles_m);
+      g.first_tile = total;
       probs[np] = g;
       first[np] = total;
       const int64_t nt = (int64_t)g.tiles_m * g.tiles_n * p.batch;
@@ -256,6 +257,7 @@ static std::vector<GGCand> gg_candidates(int dt, int precision, double mean_mn) 
     c.push_back({GG_BF16X3_64, 0});
     c.push_back({GG_BF16X3_64, 1});
     c.push_back({GG_BF16X3_64_BK64, 0});
+    c.push_back({GG_BF16X3_RUN, 0});
     if (mean_mn >= 96) c.push_back({GG_BF16X3_128, 0});
   } else if (dt == F32) {
     c.push_back({GG_F32_W32, 1});
@@ -263,6 +265,7 @@ static std::vector<GGCand> gg_candidates(int dt, int precision, double mean_mn) 
     c.push_back({GG_F32_64, 0});
     c.push_back({GG_F32_64, 1});
     c.push_back({GG_F32_64_BK32, 0});
+    c.push_back({GG_F32_RUN, 0});
     if (mean_mn >= 96) c.push_back({GG_F32_128, 0});
   } else {
     c.push_back({0, 0});
@@ -306,7 +309,22 @@ static Status gg_launch(ome_ctx* ctx, hipStream_t st, int dt, const GGCand& c,
   g.mode = dt == F32 ? epi_mode<OpsF32>(alpha, beta) : 0;
   for (int x = 0; x <= GG_XCDS; ++x) g.xcd_first[x] = e->xfirst[x];
   g.offsets32 = e->offsets32 && ctx->opt.grouped_buffers ? 1 : 0;
-  launch_grouped_gemm(dt, c.variant, g, gg_grid(ctx, dt, c, *e), st);
+  GGCand cc = c;
+  if (cc.variant == GG_F32_RUN || cc.variant == GG_BF16X3_RUN) {
+    if (dt == F32 && g.offsets32) {
+      // every workgroup resident (as many per CU as fit), no more than the widest range needs
+      int64_t widest = 0;
+      for (int x = 0; x < GG_XCDS; ++x) widest = std::max(widest, e->xfirst[x + 1] - e->xfirst[x]);
+      const int per_cu = gg_run_per_cu(cc.variant == GG_BF16X3_RUN);
+      const int64_t per = std::max<int64_t>(
+          1, std::min<int64_t>(widest, (int64_t)ctx->info.compute_units * per_cu / GG_XCDS));
+      launch_grouped_gemm(dt, cc.variant, g, (int)(per * GG_XCDS), st);
+      return Status::ok();
+    }
+    // the run kernels read through buffer resources only: the same tiles, one per workgroup
+    cc = {cc.variant == GG_F32_RUN ? GG_F32_64_BK32 : GG_BF16X3_64, 0};
+  }
+  launch_grouped_gemm(dt, cc.variant, g, gg_grid(ctx, dt, cc, *e), st);
   return Status::ok();
 }
 

@@ -18,6 +18,8 @@
 // Grouped permute: problem g writes y_g = sx*permutedims(x_g, perm_g) + sy*y_g.  Each problem
 // is a list of 64 x 64 LDS-transposed tiles (when the input's contiguous dim moves) or of
 // 1024-element runs along the output's fastest dim; the blocks of all problems form one grid.
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace ome {
@@ -515,6 +517,164 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 &&
   }
 }
 
+// ---- grouped GEMM, persistent and pipelined across tiles ----------------------------------
+// gg_run_per_cu(X3) workgroups per CU walk their XCD's tile range round-robin (workgroup j of an
+// XCD takes its tiles j, j + W, j + 2W, ...), so the tiles in flight on an XCD stay neighbours
+// and their problems' operand panels share its L2.  The K steps of a workgroup's tiles form
+// one sequence with one step in flight: the first step of the next tile is fetched as soon as
+// the current tile's C block is written, while the other workgroups of the CU run.  The next
+// tile's problem index is read a tile ahead, so a tile boundary adds one scalar read (the
+// problem's descriptor, an L2 hit) instead of a workgroup launch, a chain of descriptor
+// reads and a first operand round trip.  64 x 64 tiles, 32-deep K
+// steps, operands and C through buffer resources (the host admits only lists that fit 32-bit
+// offsets); X3: BF16x3 products, otherwise FP32 MFMA.
+template <bool X3>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gg_run_per_cu(X3), gg_run_per_cu(X3)))) void k_grouped_run(GGDesc g, const GGProb* __restrict__ probs,
+                                                     const int32_t* __restrict__ map) {
+  constexpr int TM = 64, TN = 64, BK = 32;
+  constexpr int EPT = TM * BK / 256, NBL = TN * BK / 1024;  // 8 k of one A row; 2 float4 of B
+  constexpr int LDK = X3 ? BK + 8 : BK + 4;
+  using LT = std::conditional_t<X3, uint16_t, float>;
+  __shared__ __attribute__((aligned(16))) LT sA[X3 ? 2 : 1][TM][LDK];  // [hi/lo][m][k]
+  __shared__ __attribute__((aligned(16))) LT sB[X3 ? 2 : 1][TN][LDK];  // [hi/lo][n][k]
+  const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
+  const int am = t % TM, ak = (t / TM) * EPT;
+  const float alpha = (float)g.alpha.re, beta = (float)g.beta.re;
+  const int xc = blockIdx.x % GG_XCDS;
+  const int64_t W = gridDim.x / GG_XCDS, xend = g.xcd_first[xc + 1];
+  int64_t tile = g.xcd_first[xc] + blockIdx.x / GG_XCDS;
+  if (tile >= xend) return;
+  // load side: the batch slices, sizes and tile corner whose K step lk0 is being fetched
+  // (sizes below 2^22 and slices below 2^31 bytes: 32-bit arithmetic)
+  const float *lA, *lB;
+  float* lC;
+  uint32_t lM, lN, lK, lm0, ln0, lvec;
+  auto place = [&](const GGProb& pr) __attribute__((always_inline)) {
+    const uint32_t r = (uint32_t)(tile - pr.first_tile);
+    const uint32_t b = fdiv_q(r, pr.per_div), q = r - b * pr.per_div.d;
+    const uint32_t tn = fdiv_q(q, pr.tm_div);
+    lM = (uint32_t)pr.m;
+    lN = (uint32_t)pr.n;
+    lK = (uint32_t)pr.k;
+    lm0 = (q - tn * pr.tm_div.d) * TM;
+    ln0 = tn * TN;
+    lvec = (uint32_t)pr.vec_b;
+    lA = (const float*)pr.a + (int64_t)b * lM * lK;
+    lB = (const float*)pr.b + (int64_t)b * lK * lN;
+    lC = (float*)pr.c + (int64_t)b * lM * lN;
+  };
+  place(probs[map[tile]]);
+  int64_t ntile = tile + W;  // the next tile and its problem (read a tile ahead)
+  int np = ntile < xend ? map[ntile] : 0;
+  uint32_t lk0 = 0;
+  float ra[EPT];
+  float4 rb[NBL];
+  auto load = [&]() __attribute__((always_inline)) {
+    gg_buf_load_a<EPT>(gg_rsrc(lA, lM * lK * 4u), lm0 + am < lM, lm0 + am, lk0 + ak, lM, ra);
+    gg_buf_load_b<NBL, BK>(gg_rsrc(lB, lK * lN * 4u), lvec != 0, lK, ln0, lk0, rb);
+  };
+  load();
+  // compute side: the tile whose step was fetched last (one step behind the load side)
+  float* cC;
+  uint32_t cM, cN, cm0, cn0;
+  auto take = [&]() __attribute__((always_inline)) {
+    cM = lM;
+    cN = lN;
+    cm0 = lm0;
+    cn0 = ln0;
+    cC = lC;
+  };
+  take();
+  floatx16 acc;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = 0.f;
+  for (;;) {
+    const bool last = lk0 + BK >= lK;  // the step in registers ends its tile
+    const bool more = !last || ntile < xend;
+    // the step in registers -> LDS (BF16x3: split into hi and lo halves)
+    if constexpr (X3) {
+      uint4 ph, pl;
+      split2_bf16(ra[0], ra[1], ph.x, pl.x);
+      split2_bf16(ra[2], ra[3], ph.y, pl.y);
+      split2_bf16(ra[4], ra[5], ph.z, pl.z);
+      split2_bf16(ra[6], ra[7], ph.w, pl.w);
+      *(uint4*)&sA[0][am][ak] = ph;
+      *(uint4*)&sA[1][am][ak] = pl;
+#pragma unroll
+      for (int q = 0; q < NBL; ++q) {
+        const int e = t + 256 * q;
+        const int k = (e % (BK / 4)) * 4, n = e / (BK / 4);
+        uint2 bh, bl;
+        split2_bf16(rb[q].x, rb[q].y, bh.x, bl.x);
+        split2_bf16(rb[q].z, rb[q].w, bh.y, bl.y);
+        *(uint2*)&sB[0][n][k] = bh;
+        *(uint2*)&sB[1][n][k] = bl;
+      }
+    } else {
+      *(float4*)&sA[0][am][ak] = make_float4(ra[0], ra[1], ra[2], ra[3]);
+      *(float4*)&sA[0][am][ak + 4] = make_float4(ra[4], ra[5], ra[6], ra[7]);
+#pragma unroll
+      for (int q = 0; q < NBL; ++q) {
+        const int e = t + 256 * q;
+        *(float4*)&sB[0][e / (BK / 4)][(e % (BK / 4)) * 4] = rb[q];
+      }
+    }
+    __syncthreads();
+    if (!last) {  // the next step of this tile
+      lk0 += BK;
+      load();
+    }
+    // a wave whose 32 x 32 block lies outside the problem skips its MFMAs
+    if (cm0 + wm < cM && cn0 + wn < cN) {
+      if constexpr (X3) {
+#pragma unroll
+        for (int s2 = 0; s2 < BK / 16; ++s2) {
+          const int kk = 16 * s2 + 8 * (lane >> 5);
+          const bf16x8_t ah = *(const bf16x8_t*)&sB[0][wn + (lane & 31)][kk];
+          const bf16x8_t al = *(const bf16x8_t*)&sB[1][wn + (lane & 31)][kk];
+          const bf16x8_t bh = *(const bf16x8_t*)&sA[0][wm + (lane & 31)][kk];
+          const bf16x8_t bl = *(const bf16x8_t*)&sA[1][wm + (lane & 31)][kk];
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int kb = 0; kb < BK; kb += 8) {
+          const int kk = kb + 4 * (lane >> 5);
+          const float4 af = *(const float4*)&sB[0][wn + (lane & 31)][kk];
+          const float4 bf = *(const float4*)&sA[0][wm + (lane & 31)][kk];
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(af.x, bf.x, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(af.y, bf.y, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(af.z, bf.z, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(af.w, bf.w, acc, 0, 0, 0);
+        }
+      }
+    }
+    if (last) {
+      // the C block goes out before the next tile's first step is fetched: stores and loads
+      // retire in issue order, so the wait for that step then waits for no store
+      const floatx16 (&a1)[1][1] = *(const floatx16(*)[1][1])&acc;
+      gg_buf_store_c<1, 1>(cC, cM, cN, (int64_t)(cm0 + wm), (int64_t)(cn0 + wn), lane, g.mode, alpha,
+                           beta, a1);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q] = 0.f;
+      if (more) {
+        tile = ntile;
+        place(probs[np]);
+        lk0 = 0;
+        load();
+        take();
+        ntile = tile + W;
+        if (ntile < xend) np = map[ntile];
+      }
+    }
+    __syncthreads();
+    if (!more) break;
+  }
+}
+
 // ---- grouped GEMM, one wave per 32 x 32 tile ----------------------------------------------
 // No LDS and no barriers: every wave owns a tile and loads its MFMA fragments straight from
 // global memory (A along m: 32 lanes read 128 contiguous bytes per k; B along k: 16 bytes per
@@ -791,7 +951,12 @@ void launch_grouped_gemm(int dt, int variant, const GGDesc& g, int grid, hipStre
   OME_DRY_RETURN(1);
   TimedScope ts(KC_GROUPED, st);
   if (dt == F32) {
-    if (variant == GG_F32_W32)
+    // run variants: the host launches them only when every problem fits 32-bit offsets
+    if (variant == GG_BF16X3_RUN)
+      hipLaunchKernelGGL((k_grouped_run<true>), dim3(grid), dim3(256), 0, st, g, g.probs, g.map);
+    else if (variant == GG_F32_RUN)
+      hipLaunchKernelGGL((k_grouped_run<false>), dim3(grid), dim3(256), 0, st, g, g.probs, g.map);
+    else if (variant == GG_F32_W32)
       hipLaunchKernelGGL((k_grouped_f32_wave<4>), dim3(grid), dim3(256), 0, st, g);
     else if (variant == GG_BF16X3_W32)
       hipLaunchKernelGGL((k_grouped_bf16x3_wave<2>), dim3(grid), dim3(256), 0, st, g);

@@ -154,6 +154,7 @@ struct GGProb {  // one problem of a grouped GEMM: dense column-major, batch-maj
   int32_t tiles_m, tiles_n;
   int32_t vec_a, vec_b;  // 16-byte vector loads allowed along m (A) / k (B)
   FDiv per_div, tm_div;  // invariant division by tiles_m * tiles_n and by tiles_m
+  int64_t first_tile;    // the problem's first tile in the launch's tile numbering
 };
 // Workgroup b runs on XCD b % GG_XCDS (the dispatcher deals workgroups to the XCDs in turn);
 // the tile list is cut into one contiguous range per XCD, each holding whole problems, so a
@@ -173,8 +174,12 @@ struct GGDesc {
 };
 enum GGVariant : int {
   GG_F32_64 = 0, GG_F32_64_BK32 = 1, GG_F32_128 = 2, GG_BF16X3_64 = 3, GG_BF16X3_128 = 4,
-  GG_F32_W32 = 5, GG_BF16X3_W32 = 6, GG_BF16X3_64_BK64 = 7, GG_NVARIANTS = 8
+  GG_F32_W32 = 5, GG_BF16X3_W32 = 6, GG_BF16X3_64_BK64 = 7, GG_F32_RUN = 8, GG_BF16X3_RUN = 9,
+  GG_NVARIANTS = 10
 };
+// Workgroups per CU of the run variants (persistent, one per SIMD wave slot they fit: 20 KiB
+// of LDS each; BF16x3 in 72 VGPRs, FP32 in 80).
+constexpr int gg_run_per_cu(bool x3) { return x3 ? 7 : 6; }
 void gg_tile_shape(int dt, int variant, int& tm, int& tn);
 void launch_grouped_gemm(int dt, int variant, const GGDesc& g, int grid, hipStream_t st);
 

@@ -25,6 +25,9 @@ def om():
     return omeinsum_mi355x
 
 
+SPLIT = (3, 4, 6, 7, 9)  # the tile variants with BF16x3 products
+
+
 def problems(seed, n, dtype, dims=range(1, 201), batches=range(1, 5)):
     r = rng(seed)
     out = []
@@ -71,7 +74,7 @@ def check(ps, got, alpha, beta, dtype, precision="fp32"):
 
 
 @pytest.mark.parametrize("buffers", [1, 0])
-@pytest.mark.parametrize("variant", [-1, 0, 1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [-1, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
 def test_grouped_gemm_f32_variants(variant, buffers):
     # buffers 0 forces the pointer form of the kernels that otherwise read through buffer
     # resources (the form problems above 32-bit offsets take)
@@ -81,7 +84,7 @@ def test_grouped_gemm_f32_variants(variant, buffers):
     ctx.set_option(_lib.OPT_GROUPED_BUFFERS, buffers)
     try:
         ps = problems(3100 + variant, 120, np.float32)
-        prec = "bf16x3" if variant in (3, 4, 6, 7) else "fp32"  # the split tile variants
+        prec = "bf16x3" if variant in SPLIT else "fp32"
         got = run(ps, True, False)
         check(ps, got, 1.0, 0.0, np.float32, prec)
         got = run(ps, 1.5, -0.5)
@@ -89,6 +92,27 @@ def test_grouped_gemm_f32_variants(variant, buffers):
     finally:
         ctx.set_option(_lib.OPT_GROUPED_VARIANT, -1)
         ctx.set_option(_lib.OPT_GROUPED_BUFFERS, 1)
+
+
+@pytest.mark.parametrize("variant", [8, 9])
+def test_grouped_gemm_runs_cross_tiles_and_problems(variant):
+    # about 10 000 tiles over 1536 runs: every run walks several tiles and most cross from one
+    # problem (or batch entry) into the next, carrying the next tile's first K step across
+    from omeinsum_mi355x import _lib
+    ctx = _lib.context()
+    ctx.set_option(_lib.OPT_GROUPED_VARIANT, variant)
+    try:
+        ps = problems(3120 + variant, 700, np.float32, dims=range(1, 141), batches=range(1, 13))
+        for A, B, C in ps[:3]:
+            assert A.shape[2] > 0
+        prec = "bf16x3" if variant in SPLIT else "fp32"
+        got = run(ps, True, False)
+        check(ps, got, 1.0, 0.0, np.float32, prec)
+        got = run(ps, 0.5, 2.0)
+        check(ps, got, 0.5, 2.0, np.float32, prec)
+        assert ctx.get_option(_lib.OPT_LAST_GROUPED_CHOICE) // 2 == variant
+    finally:
+        ctx.set_option(_lib.OPT_GROUPED_VARIANT, -1)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
@@ -105,7 +129,7 @@ def test_grouped_gemm_c3_shapes_autotuned_twice(prec):
             got = run(ps, True, False)
             check(ps, got, 1.0, 0.0, np.float32, prec)
             variant = ctx.get_option(_lib.OPT_LAST_GROUPED_CHOICE) // 2
-            assert (variant in (3, 4, 6, 7)) == (prec == "bf16x3"), variant
+            assert (variant in SPLIT) == (prec == "bf16x3"), variant
     finally:
         ctx.set_option(_lib.OPT_PRECISION, 0)
 
