@@ -666,6 +666,9 @@ def main():
                    help="do not run the MFMA-loop calibration program (a child process)")
     p.add_argument("--no-per-problem", action="store_true",
                    help="leave out the uneven-batch workload's one-call-per-problem timing")
+    p.add_argument("--trace-ranges", action="store_true",
+                   help="roctx ranges around tree steps, graph replays, grouped launches and "
+                        "requests (for rocprofv3 --marker-trace)")
     args = p.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this backend runs on a single MI355X (no multi-GPU "
@@ -677,6 +680,9 @@ def main():
         print("bench.py needs the GPU", file=sys.stderr)
         sys.exit(1)
     import omeinsum_mi355x as om  # noqa: F401
+    if args.trace_ranges:
+        from omeinsum_mi355x import _lib
+        _lib.context().set_option(_lib.OPT_TRACE_RANGES, 1)
 
     precs = ("bf16x3", "fp32") if args.workload in ("all", "c4") else ("bf16x3",)
     c4, ab = bench_c4(args.steps, args.warmup, precs)

@@ -99,18 +99,23 @@ typedef enum ome_option {
                                   1 64x64 with K step 32, 2 128x128, 3 BF16x3 64x64,
                                   4 BF16x3 128x128, 5 one wave per 32x32 tile,
                                   6 BF16x3 one wave per 32x32 tile, 7 BF16x3 64x64 with K
-                                  step 64, 8 64x64 persistent runs of consecutive tiles,
-                                  9 BF16x3 64x64 persistent runs (8 and 9 need buffer
-                                  resources and otherwise run as 1 and 3; Float32 only; for
-                                  tests) */
+                                  step 64, 8 64x64 persistent and pipelined across tiles,
+                                  9 BF16x3 64x64 persistent and pipelined (8 and 9 need
+                                  buffer resources and otherwise run as 1 and 3; Float32
+                                  only; for tests) */
   OME_OPT_LAST_GROUPED_CHOICE = 9, /* read-only: 2*variant + persistent of the last grouped
                                       GEMM launch (the autotuner's pick) */
   OME_OPT_TREE_DEFER_MEMORY = 10,  /* 1: ome_tree_create sizes the plan's device memory but
                                       does not allocate it (first use or the request loop
                                       does); ome_tree_get_info reports the size */
-  OME_OPT_GROUPED_BUFFERS = 11     /* 1 (default): Float32 grouped GEMMs whose problems all fit
+  OME_OPT_GROUPED_BUFFERS = 11,    /* 1 (default): Float32 grouped GEMMs whose problems all fit
                                       32-bit offsets read and write through buffer resources;
                                       0: always through pointers (for tests) */
+  OME_OPT_TRACE_RANGES = 12        /* 1: roctx ranges (process-wide) around every tree
+                                      evaluation, tree step, graph replay, grouped launch and
+                                      request (admission to retirement), for rocprofv3
+                                      --marker-trace; OME_ERR_UNSUPPORTED when the roctx
+                                      library (librocprofiler-sdk-roctx) cannot be loaded */
 } ome_option;
 
 /* Kernel classes of ome_get_kernel_timing. */
@@ -347,6 +352,14 @@ int ome_server_get_stats(ome_server* srv, ome_server_stats* stats);
  * shared by all of them, then each its other steps.  Each request computes exactly what it
  * computes alone. */
 int ome_server_set_batching(ome_server* srv, int on);
+/* Request status word (no reference counterpart; for tests of failure handling): every
+ * execution slot owns a device-visible status word, cleared when a request is admitted to it
+ * and read when the request retires; a non-zero word fails that request alone
+ * (-OME_ERR_DEVICE) while the requests beside it complete.  This call marks the next request
+ * submitted to the loop (code 0 clears the mark) so that, when it runs, a kernel on its slot
+ * stream writes `code` into the word after its steps, as a kernel that detects a fault in its
+ * own work would.  Every kernel still runs to its end. */
+int ome_server_inject_status(ome_server* srv, int code);
 
 #ifdef __cplusplus
 }

@@ -1,5 +1,7 @@
 // The C ABI (include/omeinsum_mi355x.h): context, memory pool, flat einsum, permute and
 // batched GEMM entry points.  Trees and the request loop live in tree.hip / server.hip.
+#include <dlfcn.h>
+
 #include <cstdio>
 #include <cstring>
 
@@ -176,6 +178,57 @@ void timing_collect(int kc, double* total_ms, int64_t* launches) {
   *launches = n;
 }
 
+// ---- roctx ranges ----------------------------------------------------------------------------
+// The profiler's roctx library is opened on first use (no link-time dependency); its entry
+// points are the roctx C API of rocprofiler-sdk.
+namespace {
+struct RangeApi {
+  std::mutex mu;
+  bool loaded = false, on = false;
+  int (*push)(const char*) = nullptr;
+  int (*pop)() = nullptr;
+  uint64_t (*start)(const char*) = nullptr;
+  void (*stop)(uint64_t) = nullptr;
+};
+RangeApi& range_api() {
+  static RangeApi r;
+  return r;
+}
+}  // namespace
+
+bool ranges_enable(bool on) {
+  RangeApi& r = range_api();
+  std::lock_guard<std::mutex> g(r.mu);
+  if (on && !r.loaded) {
+    void* h = dlopen("librocprofiler-sdk-roctx.so.1", RTLD_NOW | RTLD_GLOBAL);
+    if (!h) h = dlopen("librocprofiler-sdk-roctx.so", RTLD_NOW | RTLD_GLOBAL);
+    if (!h) return false;
+    r.push = (int (*)(const char*))dlsym(h, "roctxRangePushA");
+    r.pop = (int (*)())dlsym(h, "roctxRangePop");
+    r.start = (uint64_t(*)(const char*))dlsym(h, "roctxRangeStartA");
+    r.stop = (void (*)(uint64_t))dlsym(h, "roctxRangeStop");
+    if (!r.push || !r.pop || !r.start || !r.stop) return false;
+    r.loaded = true;
+  }
+  r.on = on;
+  return true;
+}
+bool ranges_enabled() { return range_api().on; }
+uint64_t range_start(const char* name) {
+  RangeApi& r = range_api();
+  return r.on ? r.start(name) : 0;
+}
+void range_stop(uint64_t id) {
+  RangeApi& r = range_api();
+  if (r.on && id) r.stop(id);
+}
+RangeScope::RangeScope(const char* name) : on(range_api().on) {
+  if (on) range_api().push(name);
+}
+RangeScope::~RangeScope() {
+  if (on) range_api().pop();
+}
+
 }  // namespace ome
 
 // ---- error plumbing -------------------------------------------------------------------------
@@ -311,6 +364,10 @@ int ome_set_option(ome_ctx* ctx, int key, int64_t v) {
     case OME_OPT_KERNEL_TIMING: timing_enable(v != 0); break;
     case OME_OPT_TREE_DEFER_MEMORY: ctx->opt.defer_tree_mem = (int)(v != 0); break;
     case OME_OPT_GROUPED_BUFFERS: ctx->opt.grouped_buffers = (int)(v != 0); break;
+    case OME_OPT_TRACE_RANGES:
+      if (!ranges_enable(v != 0))
+        return fail(ctx, OME_ERR_UNSUPPORTED, "roctx library (librocprofiler-sdk-roctx) not found");
+      break;
     case OME_OPT_GROUPED_VARIANT:
       if (v < -1 || v >= GG_NVARIANTS) return fail(ctx, OME_ERR_INVALID_ARG, "grouped variant must be -1..9");
       ctx->opt.grouped_variant = (int)v; break;
@@ -335,6 +392,7 @@ int ome_get_option(ome_ctx* ctx, int key, int64_t* v) {
     case OME_OPT_LAST_GROUPED_CHOICE: *v = ctx->last_grouped_choice; break;
     case OME_OPT_TREE_DEFER_MEMORY: *v = ctx->opt.defer_tree_mem; break;
     case OME_OPT_GROUPED_BUFFERS: *v = ctx->opt.grouped_buffers; break;
+    case OME_OPT_TRACE_RANGES: *v = ranges_enabled() ? 1 : 0; break;
     default: return fail(ctx, OME_ERR_INVALID_ARG, "unknown option");
   }
   return OME_OK;

@@ -330,6 +330,7 @@ static Status gg_launch(ome_ctx* ctx, hipStream_t st, int dt, const GGCand& c,
 
 Status run_grouped_gemm(ome_ctx* ctx, int dt, const std::vector<GGProblem>& ps, const Scal& alpha,
                         const Scal& beta, hipStream_t st) {
+  RangeScope rg("grouped GEMM");
   double flops = 0, smn = 0, sm = 0, sn = 0, sk = 0, w = 0;
   for (const auto& p : ps) {
     const double f = (double)p.m * p.n * std::max<int64_t>(p.k, 1) * p.batch;
@@ -563,6 +564,7 @@ static int64_t gp_plan(int dt, const std::vector<int64_t>& od, const std::vector
 Status run_grouped_permute(ome_ctx* ctx, int dt, int np, const int32_t* ndims, const int64_t* dims,
                            const int32_t* perms, const void* const* a, void* const* c,
                            const Scal& sx, const Scal& sy, hipStream_t st) {
+  RangeScope rg("grouped permute");
   std::vector<int64_t> key = {2, dt};
   std::vector<GPProb> probs;
   std::vector<int64_t> first;

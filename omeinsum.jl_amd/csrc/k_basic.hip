@@ -67,6 +67,18 @@ void launch_axpby(int dt, const void* x, void* y, int64_t n, const Scal& sx, con
   });
 }
 
+// One lane writes the word; the fence makes it visible to the host before the kernel ends.
+__global__ void k_status_set(int* status, int code) {
+  if (threadIdx.x == 0) {
+    *status = code;
+    __threadfence_system();
+  }
+}
+
+void launch_status_set(int* status, int code, hipStream_t st) {
+  hipLaunchKernelGGL(k_status_set, dim3(1), dim3(64), 0, st, status, code);
+}
+
 void launch_fill(int dt, void* y, int64_t n, const Scal& v, hipStream_t st) {
   if (n <= 0) return;
   OME_DRY_RETURN(1);

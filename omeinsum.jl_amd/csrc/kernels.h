@@ -39,6 +39,24 @@ struct TimedScope {
   TimedScope& operator=(const TimedScope&) = delete;
 };
 
+// roctx ranges (process-wide, off by default), resolved from the ROCm profiler's roctx library
+// when switched on.  RangeScope pushes a nested range on the calling thread; range_start /
+// range_stop bracket work that ends on another call (a request from admission to retirement).
+bool ranges_enable(bool on);  // false when the roctx library cannot be loaded
+bool ranges_enabled();
+uint64_t range_start(const char* name);  // 0 when ranges are off
+void range_stop(uint64_t id);
+struct RangeScope {
+  bool on;
+  explicit RangeScope(const char* name);
+  ~RangeScope();
+  RangeScope(const RangeScope&) = delete;
+  RangeScope& operator=(const RangeScope&) = delete;
+};
+
+// Writes code into *status (a device-visible word) from a kernel on st.
+void launch_status_set(int* status, int code, hipStream_t st);
+
 enum Prec : int { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_FP64 = 2 };
 enum GemmKernel : int { GEMM_AUTO = -1, GEMM_VALU_TILE = 0, GEMM_VALU_DIRECT = 1, GEMM_MFMA = 2 };
 

@@ -26,7 +26,10 @@
 // subtrees (the maximal subtrees of small steps a plan evaluates in one launch) together, in
 // one launch shared by all of them, before each runs its other steps on its slot stream.
 // Sliced requests (SlicedEinsum plans) keep their FP64 cross-slice accumulator in the lent
-// memory.  A tree must not be used elsewhere while its request is queued or running, and the
+// memory.  Every slot owns a status word in coherent pinned memory, cleared at admission and
+// read at retirement: a kernel that writes it non-zero fails its own request, and only that
+// one (ome_server_inject_status makes the next request write it, for tests).  With roctx ranges on,
+// each request is one range from admission to retirement.  A tree must not be used elsewhere while its request is queued or running, and the
 // context's stream must not be under graph capture while the loop has queued requests.
 #include <chrono>
 #include <condition_variable>
@@ -58,6 +61,8 @@ struct Req {
   size_t bytes = 0;     // plan memory
   size_t held = 0;      // bytes of the lent block (>= bytes)
   char* mem = nullptr;
+  int inject = 0;       // non-zero: written into the slot's status word after the steps
+  uint64_t range = 0;   // roctx range from admission to retirement
 };
 
 struct Block {
@@ -89,6 +94,11 @@ struct ome_server {
   char* job_dev[2] = {nullptr, nullptr};
   int jb = 0;
   std::vector<Slot> slots;
+  // one status word per slot in coherent pinned memory: kernels write it through status_dev,
+  // the scheduler reads it when the slot's request retires
+  int* status_host = nullptr;
+  int* status_dev = nullptr;
+  int inject_next = 0;  // status the next submitted request writes (ome_server_inject_status)
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Req*> queue;
@@ -103,6 +113,8 @@ namespace {
 
 void retire(ome_server* s, Slot& sl, int status, const std::string& msg) {
   Req* r = sl.req;
+  range_stop(r->range);
+  r->range = 0;
   tree_return_lent(r->tree);
   if (r->mem) {
     if (status == 2) {  // kept for the next admissions, tagged with its last plan
@@ -182,6 +194,7 @@ std::vector<Status> launch_round(ome_server* s, const std::vector<Slot*>& round)
       ok = ok && hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s->bst) == hipSuccess;
       ok = ok && hipEventRecord(s->copy_ev[b], s->bst) == hipSuccess;
       if (ok) {
+        RangeScope rg("request loop: shared launch of fused small steps");
         launch_fused_jobs(dt, (const FusedJob*)dev, (int)jobs.size(), nsub, s->bst);
         ok = hipGetLastError() == hipSuccess && hipEventRecord(s->batch_done, s->bst) == hipSuccess;
       }
@@ -202,10 +215,15 @@ std::vector<Status> launch_round(ome_server* s, const std::vector<Slot*>& round)
   for (size_t i = 0; i < round.size(); ++i) {
     Slot& sl = *round[i];
     Req* r = sl.req;
+    const size_t idx = (size_t)(&sl - s->slots.data());
+    // the slot's previous request has retired, so no kernel writes its word any more
+    __atomic_store_n(&s->status_host[idx], 0, __ATOMIC_RELEASE);
+    if (ranges_enabled()) r->range = range_start(("request " + std::to_string(r->ticket)).c_str());
     hipStreamWaitEvent(sl.st, r->ready, 0);
     hipStreamWaitEvent(sl.st, s->round_admit, 0);
     if (batched[i]) hipStreamWaitEvent(sl.st, s->batch_done, 0);
     Status st = tree_run_lent(r->tree, r->xs.data(), r->y, r->mem, sl.st, batched[i] != 0);
+    if (st.good() && r->inject) launch_status_set(s->status_dev + idx, r->inject, sl.st);
     const hipError_t e = hipEventRecord(sl.done, sl.st);
     if (!st.good() || e != hipSuccess) {
       if (e != hipSuccess) hipGetLastError();
@@ -228,7 +246,12 @@ void scheduler(ome_server* s) {
       const hipError_t q = hipEventQuery(sl.done);
       if (q == hipErrorNotReady) continue;
       if (q == hipSuccess) {
-        retire(s, sl, 2, "");
+        const int ks = __atomic_load_n(&s->status_host[&sl - s->slots.data()], __ATOMIC_ACQUIRE);
+        if (ks != 0)
+          retire(s, sl, -OME_ERR_DEVICE,
+                 "the request's kernels reported status " + std::to_string(ks));
+        else
+          retire(s, sl, 2, "");
       } else {
         hipGetLastError();
         retire(s, sl, -OME_ERR_DEVICE, std::string("request failed on the device: ") + hipGetErrorString(q));
@@ -351,6 +374,8 @@ void free_server_resources(ome_server* s) {
   }
   for (auto& b : s->free_blocks) s->ctx->pool->free(b.p);
   s->free_blocks.clear();
+  if (s->status_host) hipHostFree(s->status_host);
+  s->status_host = s->status_dev = nullptr;
 }
 
 int sfail(ome_server* s, int code, const std::string& msg) {
@@ -381,6 +406,10 @@ int ome_server_create(ome_ctx* ctx, size_t hbm_budget, int max_active, ome_serve
   ok = ok && hipStreamCreateWithFlags(&s->bst, hipStreamNonBlocking) == hipSuccess &&
        hipEventCreateWithFlags(&s->round_admit, hipEventDisableTiming) == hipSuccess &&
        hipEventCreateWithFlags(&s->batch_done, hipEventDisableTiming) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&s->status_host, (size_t)max_active * sizeof(int),
+                           hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
+       hipHostGetDevicePointer((void**)&s->status_dev, s->status_host, 0) == hipSuccess;
+  if (ok) std::memset(s->status_host, 0, (size_t)max_active * sizeof(int));
   for (int b = 0; ok && b < 2; ++b) {
     ok = hipEventCreateWithFlags(&s->copy_ev[b], hipEventDisableTiming) == hipSuccess &&
          hipHostMalloc((void**)&s->job_host[b], jbytes, hipHostMallocDefault) == hipSuccess;
@@ -440,6 +469,8 @@ int ome_server_submit(ome_server* s, ome_tree* tree, const void* const* xs, void
     std::lock_guard<std::mutex> lk(s->mu);
     r->ticket = s->next++;
     r->t_submit = now_ms();
+    r->inject = s->inject_next;
+    s->inject_next = 0;
     ++s->stats.submitted;
     if (fit != tree) ++s->stats.sliced;
     s->reqs[r->ticket] = r;
@@ -492,6 +523,13 @@ int ome_server_destroy(ome_server* s) {
   }
   free_server_resources(s);
   delete s;
+  return OME_OK;
+}
+
+int ome_server_inject_status(ome_server* s, int code) {
+  if (!s) return OME_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->inject_next = code;
   return OME_OK;
 }
 

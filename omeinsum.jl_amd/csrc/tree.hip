@@ -362,6 +362,7 @@ static Status run_steps(ome_tree* t, const std::vector<const void*>& xs, void* r
   // the fused subtrees first, in one launch (the normalised evaluation runs every step alone)
   const bool fuse = t->fsub > 0 && !normalize;
   if (fuse && !t->fused_elsewhere) {
+    RangeScope rg("tree: fused small steps");
     FusedLeaves lv{};
     for (int k = 0; k < t->n_leaves; ++k) lv.p[k] = xs[k];
     launch_fused_steps(t->dt, t->fdev, t->ffirst, t->fsub, lv, t->mem, st);
@@ -378,6 +379,19 @@ static Status run_steps(ome_tree* t, const std::vector<const void*>& xs, void* r
       else c.xs.push_back(t->mem + t->steps[-(a.ref + 1)].out_off);
     }
     c.iy = drop_sliced(step.iy, t->slicing);
+    std::string rname;
+    if (ranges_enabled()) {  // "tree step s: ixs -> iy" with the labels as numbers
+      rname = "tree step " + std::to_string(s) + ":";
+      for (size_t q = 0; q < c.ixs.size(); ++q) {
+        rname += q ? ", (" : " (";
+        for (size_t l = 0; l < c.ixs[q].size(); ++l) rname += (l ? " " : "") + std::to_string(c.ixs[q][l]);
+        rname += ")";
+      }
+      rname += " -> (";
+      for (size_t l = 0; l < c.iy.size(); ++l) rname += (l ? " " : "") + std::to_string(c.iy[l]);
+      rname += ")";
+    }
+    RangeScope rg(rname.c_str());
     const bool root = (s == n - 1);
     c.y = root ? root_out : (void*)(t->mem + step.out_off);
     c.sx = root ? rsx : one;
@@ -519,6 +533,7 @@ static Status launch_graphed(ome_tree* t, const std::vector<const void*>& xv, vo
   };
   auto it = std::find_if(t->graphs.begin(), t->graphs.end(), same);
   if (it == t->graphs.end()) {
+    RangeScope rg("tree: graph record");
     hipGraph_t graph;
     if (hipStreamBeginCapture(t->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
       hipGetLastError();
@@ -557,6 +572,7 @@ static Status launch_graphed(ome_tree* t, const std::vector<const void*>& xv, vo
     t->graphs.splice(t->graphs.begin(), t->graphs, it);
     it = t->graphs.begin();
   }
+  RangeScope rg("tree: graph replay");
   const hipError_t e = hipGraphLaunch(it->exec, st);
   if (e != hipSuccess)
     return Status::err(OME_ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e));
@@ -790,6 +806,7 @@ int ome_tree_execute(ome_tree* t, const void* const* xs, void* y, const ome_scal
     Status m = ensure_mem(t);
     if (!m.good()) return tree_fail(ctx, m);
   }
+  RangeScope rg("tree evaluation");
   if (!ctx->opt.use_graphs || !t->cap_stream)
     return tree_fail(ctx, enqueue_eval(t, xv, y, sx, sy, ctx->stream)) ;
   return tree_fail(ctx, launch_graphed(t, xv, y, sx, sy, ctx->stream));

@@ -17,6 +17,7 @@ OPT_PRECISION, OPT_COMPLEX_ALGO, OPT_ALLOW_LOOPS, OPT_DIRECT_GEMM = 0, 1, 2, 3
 OPT_USE_GRAPHS, OPT_AUTOTUNE, OPT_GEMM_KERNEL, OPT_KERNEL_TIMING = 4, 5, 6, 7
 OPT_GROUPED_VARIANT, OPT_LAST_GROUPED_CHOICE, OPT_TREE_DEFER_MEMORY = 8, 9, 10
 OPT_GROUPED_BUFFERS = 11
+OPT_TRACE_RANGES = 12
 # the options a tree plan freezes at creation
 PLAN_OPTIONS = (OPT_PRECISION, OPT_COMPLEX_ALGO, OPT_ALLOW_LOOPS, OPT_DIRECT_GEMM, OPT_GEMM_KERNEL)
 # kernel classes of ome_get_kernel_timing
@@ -109,6 +110,7 @@ SIGNATURES = {
     "ome_server_set_request_limit": (c_int, [c_void_p, c_size_t]),
     "ome_server_get_stats": (c_int, [c_void_p, P(ServerStats)]),
     "ome_server_set_batching": (c_int, [c_void_p, c_int]),
+    "ome_server_inject_status": (c_int, [c_void_p, c_int]),
 }
 
 _lib = None

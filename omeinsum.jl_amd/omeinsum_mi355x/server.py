@@ -64,6 +64,11 @@ class Server:
         self._live[t.value] = (plan, ts, res)
         return t.value
 
+    def inject_status(self, code):
+        """Failure-handling tests: the next submitted request writes `code` into its slot's
+        device status word after its steps, and then fails alone (wait() raises)."""
+        _lib.check(self.ctx.lib.ome_server_inject_status(self.h, int(code)), self.ctx.h)
+
     def poll(self, ticket):
         """0 queued, 1 running, 2 done, < 0 failed."""
         st = ctypes.c_int()

@@ -209,3 +209,57 @@ def test_cross_request_batching_matches_unbatched():
     for (tree, sd, xs, _), v in zip(reqs, outs[True]):
         ref = fp64(tree, sd, xs)
         assert abs(v - ref) <= 1e-5 * abs(ref), (v, ref)
+
+
+def test_injected_status_fails_one_request_alone():
+    # eight requests share four slots; the fourth writes its slot's status word after its steps
+    # (every kernel runs to its end): it fails with a device error naming the status, the seven
+    # beside it complete with their results from a loop without the injection, and the failed
+    # plan runs cleanly when submitted again
+    from omeinsum_mi355x._lib import OMEError
+    from omeinsum_mi355x.server import Server
+    reqs = [request(n, 5700 + i) for i, n in enumerate([20, 30, 24, 28, 22, 26, 32, 34])]
+    with Server(hbm_budget=1 << 30, max_active=4) as srv:
+        plans = [srv.plan(tree, dxs, sd) for tree, sd, xs, dxs in reqs]
+        clean = [complex(om().to_numpy(srv.wait(srv.submit(p, r[3]))[0])[()])
+                 for p, r in zip(plans, reqs)]
+        tickets = []
+        for i, (p, r) in enumerate(zip(plans, reqs)):
+            if i == 3:
+                srv.inject_status(7)
+            tickets.append(srv.submit(p, r[3]))
+        for i, t in enumerate(tickets):
+            if i == 3:
+                with pytest.raises(OMEError, match="status 7"):
+                    srv.wait(t)
+            else:
+                assert complex(om().to_numpy(srv.wait(t)[0])[()]) == clean[i]
+        again = complex(om().to_numpy(srv.wait(srv.submit(plans[3], reqs[3][3]))[0])[()])
+        assert again == clean[3]
+        st = srv.stats()
+        assert st["submitted"] == 2 * len(reqs) + 1
+        assert st["completed"] == 2 * len(reqs)
+
+
+def test_trace_ranges_leave_results_unchanged():
+    # roctx ranges on (tree steps, graph replays, grouped launches, requests): the same tree
+    # through a direct plan and through the loop gives the results it gives with ranges off
+    from omeinsum_mi355x import _lib
+    from omeinsum_mi355x.server import Server
+    ctx = _lib.context()
+    tree, sd, xs, dxs = request(36, 5800)
+    outs = {}
+    for on in (0, 1):
+        ctx.set_option(_lib.OPT_TRACE_RANGES, on)
+        try:
+            assert ctx.get_option(_lib.OPT_TRACE_RANGES) == on
+            with Server(hbm_budget=1 << 30, max_active=2) as srv:
+                plan = srv.plan(tree, dxs, sd)
+                looped = [complex(om().to_numpy(srv.wait(srv.submit(plan, dxs))[0])[()])
+                          for _ in range(2)]
+                direct = complex(om().to_numpy(plan.execute(dxs))[()])
+            outs[on] = (looped, direct)
+        finally:
+            ctx.set_option(_lib.OPT_TRACE_RANGES, 0)
+    assert outs[0] == outs[1]
+    assert outs[1][0][0] == outs[1][0][1]
