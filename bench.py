@@ -110,7 +110,7 @@ def bench_c4(steps, warmup, precs=("bf16x3", "fp32")):
     for prec in precs:
         om.set_einsum_backend(om.MI355XBackend(precision=prec))
         fn = lambda: om.einsum_inplace(code, (a, b), y, True, False, sd)
-        kc = "gemm_packed" if prec == "bf16x3" else "gemm_mfma"
+        kc = "gemm_packed" if prec in ("bf16x3", "bf16x6") else "gemm_mfma"
 
         def clear():
             for k in ("gemm_packed", "gemm_mfma", "pack"):
@@ -419,7 +419,8 @@ def bench_c3(steps, warmup, per_problem_path=True):
     return out
 
 
-def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8, request_limit=1 << 30):
+def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8, request_limit=1 << 30,
+             legs=("per_step", "fp32", "bf16x3")):
     """Serving stream: requests are random 3-regular graphs with an even number of vertices in
     [40, 140], bond dimension 2, ComplexF32 Gaussian tensors, greedy order (the request list of
     networks.request_stream, seed 5000).  The request loop takes the trees unsliced: a request
@@ -428,8 +429,9 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8, request_l
     sliced on the host where the largest intermediate exceeds 2^sc_budget elements (2^25
     ComplexF32 = 256 MiB).
     Inputs and plans are prepared before the timed runs (planning time reported apart).
-    Measured: the plain per-step FP32 path back to back, the request loop at saturation (every
-    request submitted at once) in FP32 and in BF16x3, and each loop under Poisson arrivals at
+    Measured: the plain per-step FP32 path back to back (after one warm-up pass, as the loops
+    get), the request loop at saturation (every request submitted at once) in FP32, BF16x3
+    and BF16x6, and each loop under Poisson arrivals at
     70% of its saturation rate (p50 / p99 latency from submission to completion).  Every request's result
     is compared with its FP64 value (tests/golden/request_stream_48.json, the oracle's
     contraction of the same network in complex128)."""
@@ -472,23 +474,29 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8, request_l
                 "worst_request": int(np.argmax(e))}
 
     # 1. the plain per-step path (one einsum per tree node from the host, new output per node)
-    om.set_einsum_backend(om.DefaultBackend())
-    om.einsum(reqs[0][5], reqs[0][3], reqs[0][2])
-    torch.cuda.synchronize()
-    per_step = []
-    lat = []
-    t0 = time.perf_counter()
-    for n, tree, sd, dxs, f, hs in reqs:
-        a = time.perf_counter()
-        per_step.append(scalar(om.einsum(hs, dxs, sd)))
-        lat.append((time.perf_counter() - a) * 1e3)
-    t_ps = time.perf_counter() - t0
-    out["per_step"] = {"req_per_s": nreq / t_ps, "p50_ms": float(np.percentile(lat, 50)),
-                       "p99_ms": float(np.percentile(lat, 99)), "tflops": flops / t_ps / 1e12,
-                       "rel_err_fp64": err_stats(per_step)}
-    log("C5 per-step path: %.1f req/s (p50 %.2f ms, p99 %.2f ms); error vs FP64 median %.1e max %.1e"
-        % (nreq / t_ps, out["per_step"]["p50_ms"], out["per_step"]["p99_ms"],
-           out["per_step"]["rel_err_fp64"]["median"], out["per_step"]["rel_err_fp64"]["max"]))
+    per_step = None
+    if "per_step" in legs:
+        om.set_einsum_backend(om.DefaultBackend())
+        # warm-up pass over every request, as the loops get one: first-use costs (pool blocks
+        # of new sizes, kernel code objects) stay out of the timed pass
+        for n, tree, sd, dxs, f, hs in reqs:
+            scalar(om.einsum(hs, dxs, sd))
+        torch.cuda.synchronize()
+        per_step = []
+        lat = []
+        t0 = time.perf_counter()
+        for n, tree, sd, dxs, f, hs in reqs:
+            a = time.perf_counter()
+            per_step.append(scalar(om.einsum(hs, dxs, sd)))
+            lat.append((time.perf_counter() - a) * 1e3)
+        t_ps = time.perf_counter() - t0
+        out["per_step"] = {"req_per_s": nreq / t_ps, "p50_ms": float(np.percentile(lat, 50)),
+                           "p99_ms": float(np.percentile(lat, 99)), "tflops": flops / t_ps / 1e12,
+                           "rel_err_fp64": err_stats(per_step)}
+        log("C5 per-step path: %.1f req/s (p50 %.2f ms, p99 %.2f ms); error vs FP64 median %.1e "
+            "max %.1e" % (nreq / t_ps, out["per_step"]["p50_ms"], out["per_step"]["p99_ms"],
+                          out["per_step"]["rel_err_fp64"]["median"],
+                          out["per_step"]["rel_err_fp64"]["max"]))
 
     def run_loop(backend):
         res_out = {}
@@ -529,19 +537,25 @@ def bench_c5(nreq=48, sc_budget=25, hbm_budget=16 << 30, max_active=8, request_l
                                   "p50_ms": float(np.percentile(lats, 50)),
                                   "p99_ms": float(np.percentile(lats, 99))}
             res_out["loop_stats"] = srv.stats()
-        res_out["speedup_req_per_s"] = sat_rps / out["per_step"]["req_per_s"]
         res_out["rel_err_fp64"] = err_stats(vals)
-        res_out["max_rel_diff_vs_per_step"] = max(abs(a - b) / abs(b) for a, b in zip(vals, per_step))
+        if per_step is not None:
+            res_out["speedup_req_per_s"] = sat_rps / out["per_step"]["req_per_s"]
+            res_out["max_rel_diff_vs_per_step"] = max(abs(a - b) / abs(b)
+                                                      for a, b in zip(vals, per_step))
         return res_out
 
     loops = (("fp32", om.MI355XBackend(precision="fp32")),
-             ("bf16x3", om.MI355XBackend(precision="bf16x3")))
+             ("bf16x3", om.MI355XBackend(precision="bf16x3")),
+             ("bf16x6", om.MI355XBackend(precision="bf16x6")),
+             ("fp32_3m", om.MI355XBackend(precision="fp32", complex_algo="3m")))
     for name, be in loops:
+        if name not in legs:
+            continue
         res_out = run_loop(be)
         out["loop_" + name] = res_out
         log("C5 request loop (%s): %.1f req/s saturated (%.2fx the per-step path); Poisson at %.1f "
             "req/s: p50 %.2f ms, p99 %.2f ms; error vs FP64 over %d requests: median %.1e max %.1e" %
-            (name, res_out["saturated"]["req_per_s"], res_out["speedup_req_per_s"],
+            (name, res_out["saturated"]["req_per_s"], res_out.get("speedup_req_per_s", 0.0),
              res_out["poisson"]["offered_req_per_s"], res_out["poisson"]["p50_ms"],
              res_out["poisson"]["p99_ms"], res_out["rel_err_fp64"]["checked"],
              res_out["rel_err_fp64"]["median"], res_out["rel_err_fp64"]["max"]))
@@ -666,6 +680,10 @@ def main():
                    help="do not run the MFMA-loop calibration program (a child process)")
     p.add_argument("--no-per-problem", action="store_true",
                    help="leave out the uneven-batch workload's one-call-per-problem timing")
+    p.add_argument("--c5-legs", default="per_step,fp32,bf16x3,bf16x6",
+                   help="comma list of the serving-stream legs to run (per_step, fp32, bf16x3, "
+                        "bf16x6, fp32_3m); "
+                        "the loops' speed-up needs per_step")
     p.add_argument("--trace-ranges", action="store_true",
                    help="roctx ranges around tree steps, graph replays, grouped launches and "
                         "requests (for rocprofv3 --marker-trace)")
@@ -684,7 +702,7 @@ def main():
         from omeinsum_mi355x import _lib
         _lib.context().set_option(_lib.OPT_TRACE_RANGES, 1)
 
-    precs = ("bf16x3", "fp32") if args.workload in ("all", "c4") else ("bf16x3",)
+    precs = ("bf16x3", "fp32", "bf16x6") if args.workload in ("all", "c4") else ("bf16x3",)
     c4, ab = bench_c4(args.steps, args.warmup, precs)
     extra = {"c4": c4}
     if args.workload in ("all", "c4"):
@@ -696,7 +714,7 @@ def main():
     if args.workload in ("all", "c3"):
         extra["c3_grouped"] = bench_c3(max(3, args.steps // 4), 1, not args.no_per_problem)
     if args.workload in ("all", "c5"):
-        extra["c5_serving"] = bench_c5()
+        extra["c5_serving"] = bench_c5(legs=tuple(args.c5_legs.split(",")))
     if args.copy_calibration:
         copy_calibration()
     cpu = None if args.no_cpu else cpu_baseline(ab)
@@ -734,7 +752,8 @@ def main():
         # is within 1e-5 on every one of the 48 requests, vs the plain per-step FP32 GPU path
         def summary(name, r):
             return {"loop": name, "req_per_s": round(r["saturated"]["req_per_s"], 1),
-                    "vs_per_step": round(r["speedup_req_per_s"], 2),
+                    "vs_per_step": (round(r["speedup_req_per_s"], 2)
+                                    if "speedup_req_per_s" in r else None),
                     "p50_ms": round(r["poisson"]["p50_ms"], 3),
                     "p99_ms": round(r["poisson"]["p99_ms"], 3),
                     "offered_req_per_s": round(r["poisson"]["offered_req_per_s"], 1),
@@ -745,12 +764,14 @@ def main():
                     "rel_err_fp64_max": r["rel_err_fp64"]["max"]}
         loops = {k[5:]: v for k, v in c5.items() if k.startswith("loop_")}
         ok = [k for k, v in loops.items() if v["rel_err_fp64"]["max"] <= 1e-5]
-        best = max(ok or loops, key=lambda k: loops[k]["saturated"]["req_per_s"])
-        line["tn_requests"] = dict(summary(best, loops[best]),
-                                   meets_1e5=best in ok,
-                                   per_step_req_per_s=round(c5["per_step"]["req_per_s"], 1),
-                                   per_step_rel_err_fp64_max=c5["per_step"]["rel_err_fp64"]["max"],
-                                   other_loops=[summary(k, v) for k, v in loops.items() if k != best])
+        if loops:
+            best = max(ok or loops, key=lambda k: loops[k]["saturated"]["req_per_s"])
+            ps = c5.get("per_step")
+            line["tn_requests"] = dict(
+                summary(best, loops[best]), meets_1e5=best in ok,
+                per_step_req_per_s=round(ps["req_per_s"], 1) if ps else None,
+                per_step_rel_err_fp64_max=ps["rel_err_fp64"]["max"] if ps else None,
+                other_loops=[summary(k, v) for k, v in loops.items() if k != best])
     if calib and kt:
         # the clock the part sustains under this kernel's MFMA load with real (random) data
         sus = calib["random_bf16x3_tflops"]

@@ -78,7 +78,10 @@ typedef enum ome_dtype {
 typedef enum ome_precision {
   OME_PREC_FP32 = 0,   /* v_mfma_f32_*_f32: FP32 operands, FP32 accumulate (default) */
   OME_PREC_BF16X3 = 1, /* a = ah + al (bf16 hi/lo); ah*bh + ah*bl + al*bh, FP32 accumulate */
-  OME_PREC_FP64 = 2    /* operands widened to FP64, v_mfma_f64 */
+  OME_PREC_FP64 = 2,   /* operands widened to FP64, v_mfma_f64 */
+  OME_PREC_BF16X6 = 3  /* a = a0 + a1 + a2 (three bf16 planes); the six products ai*bj with
+                          i + j <= 2, FP32 accumulate: FP32-level error on the bf16 matrix
+                          cores (large products; smaller ones run at OME_PREC_FP32) */
 } ome_precision;
 
 typedef enum ome_complex_algo {

@@ -351,7 +351,7 @@ int ome_set_option(ome_ctx* ctx, int key, int64_t v) {
   std::lock_guard<std::recursive_mutex> g(ctx->mu);
   switch (key) {
     case OME_OPT_PRECISION:
-      if (v < 0 || v > 2) return fail(ctx, OME_ERR_INVALID_ARG, "precision must be 0, 1 or 2");
+      if (v < 0 || v > 3) return fail(ctx, OME_ERR_INVALID_ARG, "precision must be 0, 1, 2 or 3");
       ctx->opt.precision = (int)v; break;
     case OME_OPT_COMPLEX_ALGO: ctx->opt.complex_algo = (int)(v != 0); break;
     case OME_OPT_ALLOW_LOOPS: ctx->opt.allow_loops = (int)(v != 0); break;

@@ -268,6 +268,27 @@ __device__ __forceinline__ void split2_bf16(float a, float b, uint32_t& hi, uint
   hi = hb;
 }
 
+// Three-plane split of two floats (six-product form): hi = bf16(x), mid = bf16(x - hi),
+// lo = bf16(x - hi - mid), each rounded to nearest even.  Both differences are exact in FP32
+// (each subtrahend is the nearest bf16 of its minuend), so |x - hi - mid - lo| <= 2^-24 |x|.
+__device__ __forceinline__ void split3_bf16(float a, float b, uint32_t& hi, uint32_t& mid,
+                                            uint32_t& lo) {
+  const split_f2 x = {a, b};
+  const split_b2 h = __builtin_convertvector(x, split_b2);
+  uint32_t hb;
+  __builtin_memcpy(&hb, &h, 4);
+  const split_f2 hf = {__uint_as_float(hb << 16), __uint_as_float(hb & 0xffff0000u)};
+  const split_f2 r = x - hf;
+  const split_b2 m = __builtin_convertvector(r, split_b2);
+  uint32_t mb;
+  __builtin_memcpy(&mb, &m, 4);
+  const split_f2 mf = {__uint_as_float(mb << 16), __uint_as_float(mb & 0xffff0000u)};
+  const split_b2 l = __builtin_convertvector(r - mf, split_b2);
+  __builtin_memcpy(&lo, &l, 4);
+  hi = hb;
+  mid = mb;
+}
+
 // Epilogue mode:  y <- sx*v (sy zero: y not read) | sy*y + sx*v | sy*y (sx zero).
 enum EpiMode : int { EPI_OVERWRITE = 0, EPI_AXPBY = 1, EPI_SCALE_ONLY = 2 };
 

@@ -1,6 +1,8 @@
 // Bandwidth kernels: elementwise scaling/fill/conj/accumulate, the strided permute
 // (tensorpermute!) and the generic index walk (loop_einsum!) that also carries the unary
 // rules Diag / Sum / Repeat / Duplicate / Tr in one pass.
+#include <cstdlib>
+
 #include "kernels.h"
 
 namespace ome {
@@ -326,10 +328,170 @@ __global__ __launch_bounds__(256) void k_permute_tiled(PermDesc d, int t) {
   }
 }
 
+// Permutation of a tensor whose grouped extents are all powers of two (bond-dimension-2
+// networks: every label of extent 2).  Element offsets are then bit strings, and the
+// permutation maps output offset bits to input offset bits.  A workgroup moves one tile: the
+// elements that agree on every bit outside a set T of input bits, where T holds the input's
+// lowest bits (a contiguous run to read) and the images of the output's lowest bits (a
+// contiguous run to write), 2^8 .. 2^12 elements.  Read phase: local index i (T's bits in
+// input order) = thread + 256 k, so 64 lanes read a run; write phase: local index j (T's bits
+// in output order), so 64 lanes write a run, taking element i(j) from LDS.  Every offset is a
+// sum of per-bit weights over disjoint bits, so a thread's part (its 8 index bits), an
+// iteration's part and the tile's part are added.  LDS slots are xor-swizzled,
+// slot = i ^ (((i >> 5) ^ (i >> 10)) & 31), so the write phase's gathered reads spread over
+// the banks.
+struct BitPermDesc {
+  const void* x;
+  void* y;
+  int tbits;                 // |T| (8 .. 12)
+  int rbits;                 // bits outside T: 2^rbits tiles
+  int64_t t_in[8], t_out[8];  // thread bit q -> input offset (read phase), output offset
+  int32_t t_lds[8];           // thread bit q -> input-order local index (write phase)
+  int64_t k_in[16], k_out[16];  // iteration k -> input / output offset of its index bits
+  int32_t k_lds[16];
+  int64_t r_in[32], r_out[32];  // tile bit -> input / output offset
+  Scal sx, sy;
+  int mode;
+  int conj;
+};
+
+__device__ __forceinline__ int bp_swz(int i) { return i ^ (((i >> 5) ^ (i >> 10)) & 31); }
+
+template <class O>
+__global__ __launch_bounds__(256) void k_permute_bits(BitPermDesc d) {
+  using T = typename O::T;
+  constexpr int CAP = sizeof(T) > 8 ? 2048 : 4096;
+  __shared__ T tile[CAP];
+  const T* __restrict__ x = (const T*)d.x;
+  T* __restrict__ y = (T*)d.y;
+  const T sx = O::from(d.sx), sy = O::from(d.sy);
+  const int t = threadIdx.x;
+  uint32_t b = blockIdx.x;
+  int64_t ib = 0, ob = 0;
+  for (int r = 0; r < d.rbits; ++r)
+    if ((b >> r) & 1u) { ib += d.r_in[r]; ob += d.r_out[r]; }
+  int64_t ti = 0, to = 0;
+  int tl = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if ((t >> q) & 1) { ti += d.t_in[q]; to += d.t_out[q]; tl += d.t_lds[q]; }
+  const int iters = 1 << (d.tbits - 8);
+  T v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+    if (k < iters) v[k] = x[ib + ti + d.k_in[k]];
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+    if (k < iters) tile[bp_swz(t + 256 * k)] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+    if (k < iters) {
+      const int64_t o = ob + to + d.k_out[k];
+      const T e = conj_if<O>(tile[bp_swz(tl + d.k_lds[k])], d.conj);
+      y[o] = epilogue<O>(d.mode, sx, sy, e, y + o);
+    }
+}
+
+// The bit-permutation form of d, or false when some extent is not a power of two or the
+// tensor has fewer than 2^8 elements.
+static bool bit_perm_desc(int dt, const PermDesc& d, BitPermDesc& bp) {
+  if (d.n < 256 || d.n >= (1ll << 32)) return false;
+  int ib_of_ob[32];  // output bit -> input bit
+  int nb = 0;
+  for (int p = 0; p < d.rank; ++p) {
+    const uint32_t e = d.odiv[p].d;
+    const int64_t s = d.istride[p];
+    if (e == 0 || (e & (e - 1)) || s <= 0 || (s & (s - 1))) return false;
+    const int eb = __builtin_ctz(e), sb = __builtin_ctzll((uint64_t)s);
+    for (int j = 0; j < eb; ++j) {
+      if (nb >= 32) return false;
+      ib_of_ob[nb++] = sb + j;
+    }
+  }
+  if ((1ll << nb) != d.n) return false;
+  int ob_of_ib[32];
+  for (int j = 0; j < nb; ++j) ob_of_ib[j] = -1;
+  for (int j = 0; j < nb; ++j) {
+    if (ib_of_ob[j] >= nb || ob_of_ib[ib_of_ob[j]] >= 0) return false;
+    ob_of_ib[ib_of_ob[j]] = j;
+  }
+  const int esz = (int)elem_size(dt);
+  const int tmax = esz > 8 ? 11 : 12;
+  const int target = nb < tmax ? nb : tmax;
+  // runs of 128 bytes where the extents allow, then grow both runs alternately to the tile
+  int run = 0;
+  while ((esz << (run + 1)) <= 128) ++run;
+  bool inT[32] = {false};
+  int tcount = 0, a = 0, c = 0;
+  auto add = [&](int ib) { if (!inT[ib] && tcount < target) { inT[ib] = true; ++tcount; } };
+  while (a < run && a < nb) add(a++);
+  while (c < run && c < nb) add(ib_of_ob[c++]);
+  while (tcount < target) {
+    if (a < nb) add(a++);
+    if (c < nb) add(ib_of_ob[c++]);
+  }
+  int Tin[32], Tout[32], R[32], nt = 0, nr = 0, no = 0;
+  int pos_in[32];
+  for (int j = 0; j < nb; ++j) {
+    if (inT[j]) { pos_in[j] = nt; Tin[nt++] = j; }
+    else R[nr++] = j;
+  }
+  for (int ob = 0; ob < nb; ++ob)
+    if (inT[ib_of_ob[ob]]) Tout[no++] = ib_of_ob[ob];
+  bp = BitPermDesc{};
+  bp.x = d.x;
+  bp.y = d.y;
+  bp.tbits = nt;
+  bp.rbits = nr;
+  for (int q = 0; q < 8; ++q) {
+    bp.t_in[q] = 1ll << Tin[q];
+    bp.t_out[q] = 1ll << ob_of_ib[Tout[q]];
+    bp.t_lds[q] = 1 << pos_in[Tout[q]];
+  }
+  for (int k = 0; k < (1 << (nt - 8)); ++k) {
+    int64_t ki = 0, ko = 0;
+    int32_t kl = 0;
+    for (int q = 8; q < nt; ++q)
+      if ((k >> (q - 8)) & 1) {
+        ki += 1ll << Tin[q];
+        ko += 1ll << ob_of_ib[Tout[q]];
+        kl += 1 << pos_in[Tout[q]];
+      }
+    bp.k_in[k] = ki;
+    bp.k_out[k] = ko;
+    bp.k_lds[k] = kl;
+  }
+  for (int r = 0; r < nr; ++r) {
+    bp.r_in[r] = 1ll << R[r];
+    bp.r_out[r] = 1ll << ob_of_ib[R[r]];
+  }
+  bp.sx = d.sx;
+  bp.sy = d.sy;
+  bp.mode = d.mode;
+  bp.conj = d.conj;
+  return true;
+}
+
 void launch_permute(int dt, const PermDesc& d, hipStream_t st) {
   if (d.n <= 0) return;
   OME_DRY_RETURN(1);
   TimedScope ts(KC_PERMUTE, st);
+  // power-of-two extents (all of a bond-dimension-2 network's tensors): the bit-permutation
+  // tile.  OME_PERMUTE_BITS=0 turns it off for measurements.
+  static int bits_on = -1;
+  if (bits_on < 0) {
+    const char* e = getenv("OME_PERMUTE_BITS");
+    bits_on = (e && e[0] == '0') ? 0 : 1;
+  }
+  BitPermDesc bp;
+  if (bits_on && bit_perm_desc(dt, d, bp)) {
+    const uint32_t blocks = 1u << bp.rbits;
+    OME_DISPATCH(dt, O, {
+      hipLaunchKernelGGL(k_permute_bits<O>, dim3(blocks), dim3(256), 0, st, bp);
+    });
+    return;
+  }
   // tiled path when some output dim t > 0 is the input's contiguous dim and both extents
   // are large enough to fill a tile
   int t = -1;

@@ -342,6 +342,36 @@ __global__ __launch_bounds__(256) void k_dot_reduce(GemmDesc d) {
   }
 }
 
+// The same for many parts per output (a few outputs over a very long K): one workgroup per
+// output; thread t sums parts t, t + 256, ... in order, then a fixed LDS tree, so the 256
+// partial loads of a round are in flight together instead of one dependent chain.
+template <class O>
+__global__ __launch_bounds__(256) void k_dot_reduce_wg(GemmDesc d) {
+  using T = typename O::T;
+  using W = AccOps<O>;
+  using A = typename W::A;
+  using AT = typename A::T;
+  __shared__ AT red[256];
+  const int t = threadIdx.x;
+  const int64_t per = d.M * d.N, total = per * d.B;
+  const int64_t e = blockIdx.x;
+  const AT* Wk = (const AT*)d.work;
+  AT acc = A::zero();
+  for (int s = t; s < d.ksplit; s += 256) acc = A::add(acc, Wk[(int64_t)s * total + e]);
+  red[t] = acc;
+  __syncthreads();
+#pragma unroll
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) red[t] = A::add(red[t], red[t + s]);
+    __syncthreads();
+  }
+  if (t != 0) return;
+  const int64_t batch = e / per, r = e - batch * per;
+  const int64_t n = r / d.M, m = r - n * d.M;
+  T* yp = (T*)d.y + batch * d.syb + m * d.sym + n * d.syn;
+  *yp = epilogue<O>(d.mode, O::from(d.sx), O::from(d.sy), W::narrow(red[0]), yp);
+}
+
 // Shuffle of any trivially copyable value across the lanes of a wave, word by word.
 template <class V>
 __device__ __forceinline__ V shfl_xor_any(V v, int mask) {
@@ -524,7 +554,10 @@ void launch_gemm_dot(int dt, GemmDesc d, int parts, hipStream_t st) {
     }
     if (parts > 1) {
       TimedScope ts(KC_OTHER, st);
-      hipLaunchKernelGGL(k_dot_reduce<O>, dim3(grid_cap(outputs)), dim3(256), 0, st, d);
+      if (parts >= 32)
+        hipLaunchKernelGGL(k_dot_reduce_wg<O>, dim3((uint32_t)outputs), dim3(256), 0, st, d);
+      else
+        hipLaunchKernelGGL(k_dot_reduce<O>, dim3(grid_cap(outputs)), dim3(256), 0, st, d);
     }
   });
 }

@@ -22,7 +22,8 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int PK_K = 32;          // k per packed tile
-__host__ __device__ constexpr int pk_tile_elems(int rows) { return 2 * rows * PK_K; }
+// elements of one packed (row tile, k tile) block: np bf16 planes of rows x 32
+__host__ __device__ constexpr int pk_tile_elems(int rows, int np = 2) { return np * rows * PK_K; }
 
 // ---- pack ----------------------------------------------------------------------------------
 // One workgroup per packed tile.  The 128x32 source tile is staged through LDS so that the
@@ -73,8 +74,8 @@ __device__ __forceinline__ const float* pack_addr(const PackDesc& d, const float
 __device__ __forceinline__ int pk_swz(int k) { return ((k >> 3) & 3) << 2; }
 static_assert(PK_K == 32, "the staging tile holds 32 k per row");
 
-// TR rows per packed tile (128 or 256).
-template <int TR>
+// TR rows per packed tile (128 or 256); NP bf16 planes (2: hi, lo; 3: hi, mid, lo).
+template <int TR, int NP>
 __device__ __forceinline__ void pack_tile(const PackDesc& d, int64_t bid, float (*tile)[PK_K + 1]) {
   const int64_t tk = bid % d.tiles_k; bid /= d.tiles_k;
   const int64_t tr = bid % d.tiles_r; bid /= d.tiles_r;
@@ -149,8 +150,8 @@ __device__ __forceinline__ void pack_tile(const PackDesc& d, int64_t bid, float 
     }
   }
   __syncthreads();
-  uint16_t* out = d.dst + ((batch * d.tiles_r + tr) * d.tiles_k + tk) * (int64_t)pk_tile_elems(TR);
-  // TR*4 chunks of 8 bf16 per plane; thread t emits chunks t + 256q of both planes
+  uint16_t* out = d.dst + ((batch * d.tiles_r + tr) * d.tiles_k + tk) * (int64_t)pk_tile_elems(TR, NP);
+  // TR*4 chunks of 8 bf16 per plane; thread t emits chunks t + 256q of every plane
 #pragma unroll
   for (int q = 0; q < TR / 64; ++q) {
     const int c = t + 256 * q;
@@ -161,30 +162,49 @@ __device__ __forceinline__ void pack_tile(const PackDesc& d, int64_t bid, float 
       const int k = kq + e;
       x[e] = kmaj ? tt[k][4 * ((r >> 2) ^ pk_swz(k)) + (r & 3)] : tile[r][k];
     }
-    uint4 vh, vl;
-    split2_bf16(x[0], x[1], vh.x, vl.x);
-    split2_bf16(x[2], x[3], vh.y, vl.y);
-    split2_bf16(x[4], x[5], vh.z, vl.z);
-    split2_bf16(x[6], x[7], vh.w, vl.w);
     // row r's 16-byte chunk j lives at slot j ^ ((r >> 2) & 3): conflict-free b128 reads
     const int slot = r * 4 + ((c & 3) ^ ((r >> 2) & 3));
-    *(uint4*)(out + slot * 8) = vh;
-    *(uint4*)(out + TR * PK_K + slot * 8) = vl;
+    if constexpr (NP == 2) {
+      uint4 vh, vl;
+      split2_bf16(x[0], x[1], vh.x, vl.x);
+      split2_bf16(x[2], x[3], vh.y, vl.y);
+      split2_bf16(x[4], x[5], vh.z, vl.z);
+      split2_bf16(x[6], x[7], vh.w, vl.w);
+      *(uint4*)(out + slot * 8) = vh;
+      *(uint4*)(out + TR * PK_K + slot * 8) = vl;
+    } else {
+      uint4 vh, vm, vl;
+      split3_bf16(x[0], x[1], vh.x, vm.x, vl.x);
+      split3_bf16(x[2], x[3], vh.y, vm.y, vl.y);
+      split3_bf16(x[4], x[5], vh.z, vm.z, vl.z);
+      split3_bf16(x[6], x[7], vh.w, vm.w, vl.w);
+      *(uint4*)(out + slot * 8) = vh;
+      *(uint4*)(out + TR * PK_K + slot * 8) = vm;
+      *(uint4*)(out + 2 * TR * PK_K + slot * 8) = vl;
+    }
   }
 }
 
 template <int TR>
 __global__ __launch_bounds__(256) void k_pack_bf16x3(PackDesc d) {
   __shared__ float tile[TR][PK_K + 1];
-  pack_tile<TR>(d, blockIdx.x, tile);
+  pack_tile<TR, 2>(d, blockIdx.x, tile);
 }
 
 // Both operands of one GEMM in one launch: blocks [0, na) pack a, the rest pack b.
 template <int TR>
 __global__ __launch_bounds__(256) void k_pack2_bf16x3(PackDesc a, PackDesc b, int64_t na) {
   __shared__ float tile[TR][PK_K + 1];
-  if ((int64_t)blockIdx.x < na) pack_tile<TR>(a, blockIdx.x, tile);
-  else pack_tile<TR>(b, (int64_t)blockIdx.x - na, tile);
+  if ((int64_t)blockIdx.x < na) pack_tile<TR, 2>(a, blockIdx.x, tile);
+  else pack_tile<TR, 2>(b, (int64_t)blockIdx.x - na, tile);
+}
+
+// Three-plane pack of both operands in one launch: a in TRA-row tiles, b in TRB-row tiles.
+template <int TRA, int TRB>
+__global__ __launch_bounds__(256) void k_pack2_bf16x6(PackDesc a, PackDesc b, int64_t na) {
+  __shared__ float tile[TRA > TRB ? TRA : TRB][PK_K + 1];
+  if ((int64_t)blockIdx.x < na) pack_tile<TRA, 3>(a, blockIdx.x, tile);
+  else pack_tile<TRB, 3>(b, (int64_t)blockIdx.x - na, tile);
 }
 
 // ---- GEMM on packed operands ---------------------------------------------------------------
@@ -619,6 +639,185 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_packed_bf16x3_pipe(Packed
       }
 }
 
+// Six-product form on three planes (BF16x6): the same pipeline as the kernel above, with
+// the packed tiles holding planes 0 (hi), 1 (mid), 2 (lo) and every fragment set read for
+// all three.  Per 32 x 32 block and 16-k sub-step it issues the six products a_p * b_q with
+// p + q <= 2, smallest first: (2,0) (1,1) (0,2) (1,0) (0,1) (0,0).  The dropped terms are
+// below 2^-23 |a||b| per product, the FP32 rounding scale.
+// Tiles TM x TN = 256 x 128: a stage holds (256 + 128) x 32 k x 3 planes (72 KiB), two stages
+// fit the LDS; WM x WN = 4 x 2 waves of 64 x 64.  A stage's 72 1-KiB copies split 9 per wave,
+// so one wave's copies straddle the A / B boundary: the source is chosen per copy.
+template <int TM, int TN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_packed_bf16x6_pipe(PackedGemmDesc d) {
+  constexpr int NP = 3;
+  constexpr int NW = WM * WN;
+  constexpr int MB = TM / WM / 32, NB = TN / WN / 32;
+  constexpr int A_PLANE = TM * PK_K * 2, B_PLANE = TN * PK_K * 2;
+  constexpr int A_BYTES = NP * A_PLANE, B_BYTES = NP * B_PLANE;
+  constexpr int STAGE = A_BYTES + B_BYTES;
+  constexpr int CHUNKS = STAGE / 1024;
+  constexpr int CPW = CHUNKS / NW;
+  constexpr int A_CH = A_BYTES / 1024;
+  static_assert(CHUNKS % NW == 0, "copies must split evenly over the waves");
+  static_assert(2 * STAGE <= 160 * 1024, "two stages must fit the LDS");
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * STAGE];
+  int64_t tm, tn, batch, ks;
+  pk_map(blockIdx.x, gridDim.x, d, tm, tn, batch, ks);
+  const int t = threadIdx.x;
+  const int lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wm = (w % WM) * (TM / WM), wn = (w / WM) * (TN / WN);
+  const int64_t kt0 = ks * d.kt_per_split;
+  int64_t kt1 = kt0 + d.kt_per_split;
+  if (kt1 > d.tiles_k) kt1 = d.tiles_k;
+  const uint8_t* ga = (const uint8_t*)(d.a + ((batch * d.tiles_m + tm) * d.tiles_k) * (int64_t)pk_tile_elems(TM, NP));
+  const uint8_t* gb = (const uint8_t*)(d.b + ((batch * d.tiles_n + tn) * d.tiles_k) * (int64_t)pk_tile_elems(TN, NP));
+  const int c0 = w * CPW;
+  auto issue = [&](int64_t kt) __attribute__((always_inline)) {
+    const int sbase = (int)(kt & 1) * STAGE;
+    const uint8_t* sa = ga + kt * A_BYTES + lane * 16;
+    const uint8_t* sbb = gb + kt * B_BYTES + lane * 16;
+#pragma unroll
+    for (int i = 0; i < CPW; ++i) {
+      const int c = c0 + i;
+      const uint8_t* src = c < A_CH ? sa + c * 1024 : sbb + (c - A_CH) * 1024;
+      const int loff = __builtin_amdgcn_readfirstlane(sbase + c * 1024);
+      __builtin_amdgcn_global_load_lds((const void*)src,
+                                       (__attribute__((address_space(3))) void*)(lds + loff),
+                                       16, 0, 0);
+    }
+  };
+  const int jl = lane >> 5;
+  // fragments of sub-step h of the step in stage sb: fn[p][nb] from the B tile (rows n, the
+  // MFMA's first operand), fm[p][mb] from the A tile (rows m), plane p
+  auto rd = [&](int sb, int h, bf16x8_t (&fn)[NP][NB], bf16x8_t (&fm)[NP][MB])
+      __attribute__((always_inline)) {
+    const int base = sb * STAGE;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int r = wn + nb * 32 + (lane & 31);
+      const int off = base + A_BYTES + r * 64 + 16 * ((2 * h + jl) ^ ((r >> 2) & 3));
+#pragma unroll
+      for (int p = 0; p < NP; ++p) fn[p][nb] = *(const bf16x8_t*)(lds + off + p * B_PLANE);
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      const int r = wm + mb * 32 + (lane & 31);
+      const int off = base + r * 64 + 16 * ((2 * h + jl) ^ ((r >> 2) & 3));
+#pragma unroll
+      for (int p = 0; p < NP; ++p) fm[p][mb] = *(const bf16x8_t*)(lds + off + p * A_PLANE);
+    }
+  };
+  floatx16 acc[NB][MB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int j = 0; j < MB; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  auto mm = [&](const bf16x8_t (&fn)[NP][NB], const bf16x8_t (&fm)[NP][MB])
+      __attribute__((always_inline)) {
+    constexpr int PN[6] = {2, 1, 0, 1, 0, 0}, PM[6] = {0, 1, 2, 0, 1, 0};
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+          acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fn[PN[q]][nb], fm[PM[q]][mb],
+                                                                acc[nb][mb], 0, 0, 0);
+  };
+  bf16x8_t fn0[NP][NB], fm0[NP][MB], fn1[NP][NB], fm1[NP][MB];
+  if (kt0 < kt1) {
+    issue(kt0);
+    if (kt0 + 1 < kt1) issue(kt0 + 1);
+    if (kt0 + 1 < kt1) __builtin_amdgcn_s_waitcnt(waitcnt_vm(CPW));
+    else __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    raw_barrier();
+    rd((int)(kt0 & 1), 0, fn0, fm0);
+  }
+  constexpr int N_MFMA = 6 * NB * MB, N_RD = NP * (NB + MB);
+  int64_t kt = kt0;
+  for (; kt + 2 < kt1; ++kt) {
+    const int sb = (int)(kt & 1);
+    __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
+    __builtin_amdgcn_sched_barrier(0);
+    rd(sb, 1, fn1, fm1);
+    __builtin_amdgcn_sched_barrier(0);
+    mm(fn0, fm0);
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0) & WAIT_LGKM0);
+    raw_barrier();
+    issue(kt + 2);
+    rd(sb ^ 1, 0, fn0, fm0);
+    mm(fn1, fm1);
+    sched_interleave<N_MFMA, CPW, N_RD>();
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  for (; kt < kt1; ++kt) {
+    const int sb = (int)(kt & 1);
+    __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
+    __builtin_amdgcn_sched_barrier(0);
+    rd(sb, 1, fn1, fm1);
+    __builtin_amdgcn_sched_barrier(0);
+    mm(fn0, fm0);
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0) & WAIT_LGKM0);
+    raw_barrier();
+    if (kt + 1 < kt1) rd(sb ^ 1, 0, fn0, fm0);
+    __builtin_amdgcn_sched_barrier(0);
+    mm(fn1, fm1);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const int64_t m0 = tm * TM + wm, n0 = tn * TN + wn;
+  if (d.work) {
+    float* W = d.work + ((ks * d.B + batch) * d.M) * d.N;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t m = m0 + mb * 32 + (lane & 31);
+          const int64_t n = n0 + nb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          if (m < d.M && n < d.N) W[m + n * d.M] = acc[nb][mb][r];
+        }
+    return;
+  }
+  if (!d.cplx) {
+    const float sx = (float)d.sx.re, sy = (float)d.sy.re;
+    float* Y = (float*)d.y + batch * d.syb;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t m = m0 + mb * 32 + (lane & 31);
+          const int64_t n = n0 + nb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          if (m < d.M && n < d.N) {
+            float* yp = Y + m * d.sym + n * d.syn;
+            *yp = epilogue<OpsF32>(d.mode, sx, sy, acc[nb][mb][r], yp);
+          }
+        }
+    return;
+  }
+  float* Y = (float*)d.y + 2 * batch * d.syb;
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t m = m0 + mb * 32 + (lane & 31);
+        const int64_t n = n0 + nb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (m < d.M && n < d.N) {
+          const bool im = n >= d.N0;
+          const int64_t nn = im ? n - d.N0 : n;
+          Y[2 * (m * d.sym + nn * d.syn) + (im ? 1 : 0)] = acc[nb][mb][r];
+        }
+      }
+}
+
 // Split-K reduction for the packed GEMM (real, or complex with column halves re / im).
 __global__ __launch_bounds__(256) void k_packed_reduce(PackedGemmDesc d) {
   const int64_t per = d.M * d.N, total = per * d.B;
@@ -655,7 +854,13 @@ bool packed_plain_overwrite(const GemmDesc& g) {
 
 // Workgroup tile (TM x TN) for a packed GEMM.  OME_PACKED_TILE=128|256x128|128x256|256|256x256
 // forces one for measurements.
-static void packed_tiles(int64_t M, int64_t N, int& TM, int& TN) {
+static void packed_tiles(int64_t M, int64_t N, int& TM, int& TN, int np) {
+  // three planes: one tile shape (two stages of 256 x 128 fill the LDS)
+  if (np == 3) {
+    TM = 256;
+    TN = 128;
+    return;
+  }
   static int forced = -1;
   if (forced < 0) {
     const char* e = getenv("OME_PACKED_TILE");
@@ -686,19 +891,19 @@ static void packed_tiles(int64_t M, int64_t N, int& TM, int& TN) {
   TN = 128;
 }
 
-int64_t packed_bytes_a(int64_t M, int64_t N, int64_t K, int64_t batch) {
+int64_t packed_bytes_a(int64_t M, int64_t N, int64_t K, int64_t batch, int np) {
   int TM, TN;
-  packed_tiles(M, N, TM, TN);
-  return batch * ((M + TM - 1) / TM) * ((K + PK_K - 1) / PK_K) * (int64_t)pk_tile_elems(TM) * 2;
+  packed_tiles(M, N, TM, TN, np);
+  return batch * ((M + TM - 1) / TM) * ((K + PK_K - 1) / PK_K) * (int64_t)pk_tile_elems(TM, np) * 2;
 }
-int64_t packed_bytes_b(int64_t M, int64_t N, int64_t K, int64_t batch) {
+int64_t packed_bytes_b(int64_t M, int64_t N, int64_t K, int64_t batch, int np) {
   int TM, TN;
-  packed_tiles(M, N, TM, TN);
-  return batch * ((N + TN - 1) / TN) * ((K + PK_K - 1) / PK_K) * (int64_t)pk_tile_elems(TN) * 2;
+  packed_tiles(M, N, TM, TN, np);
+  return batch * ((N + TN - 1) / TN) * ((K + PK_K - 1) / PK_K) * (int64_t)pk_tile_elems(TN, np) * 2;
 }
-int64_t packed_tiles_mn(int64_t M, int64_t N) {
+int64_t packed_tiles_mn(int64_t M, int64_t N, int np) {
   int TM, TN;
-  packed_tiles(M, N, TM, TN);
+  packed_tiles(M, N, TM, TN, np);
   return ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
 }
 
@@ -737,20 +942,26 @@ static void launch_pack2(const PackDesc& a, const PackDesc& b, int64_t B, int TR
     hipLaunchKernelGGL(k_pack2_bf16x3<128>, dim3((uint32_t)(na + nb)), dim3(256), 0, st, a, b, na);
 }
 
-// Real F32 or complex C32 GEMM on packed BF16x3 operands.  pa/pb: workspace of
-// packed_bytes(...) each; work: split-K partials (ksplit*B*M'*N' floats) or null.
+// Real F32 or complex C32 GEMM on packed operands of np bf16 planes (2: BF16x3, 3: BF16x6).
+// pa/pb: workspace of packed_bytes(...) each; work: split-K partials (ksplit*B*M'*N' floats)
+// or null.
 void launch_gemm_packed(int dt, const GemmDesc& g, void* pa, void* pb, float* work, int ksplit,
-                        hipStream_t st) {
+                        hipStream_t st, int np) {
   const bool cplx = (dt == C32);
   PackedGemmDesc d{};
   const int64_t M = g.M, N = cplx ? 2 * g.N : g.N, K = cplx ? 2 * g.K : g.K;
   int TM, TN;
-  packed_tiles(M, N, TM, TN);
-  OME_DRY_RETURN((ksplit > 1 ? 2 : 1) + (TM == TN ? 1 : 2));
+  packed_tiles(M, N, TM, TN, np);
+  OME_DRY_RETURN((ksplit > 1 ? 2 : 1) + (TM == TN || np == 3 ? 1 : 2));
   // A' rows m (stride sam), k (stride sak); B' rows n (stride sbn), k (stride sbk)
   const PackDesc pka = pack_desc(g.a, (uint16_t*)pa, M, K, g.sam, g.sak, g.sab, cplx ? 1 : 0, g.M, g.K, TM);
   const PackDesc pkb = pack_desc(g.b, (uint16_t*)pb, N, K, g.sbn, g.sbk, g.sbb, cplx ? 2 : 0, g.N, g.K, TN);
-  if (TM == TN) {
+  if (np == 3) {
+    const int64_t na = pka.tiles_r * pka.tiles_k * g.B, nb = pkb.tiles_r * pkb.tiles_k * g.B;
+    TimedScope ts(KC_PACK, st);
+    hipLaunchKernelGGL((k_pack2_bf16x6<256, 128>), dim3((uint32_t)(na + nb)), dim3(256), 0, st,
+                       pka, pkb, na);
+  } else if (TM == TN) {
     launch_pack2(pka, pkb, g.B, TM, st);
   } else {
     launch_pack(pka, g.B, TM, st);
@@ -793,7 +1004,9 @@ void launch_gemm_packed(int dt, const GemmDesc& g, void* pa, void* pb, float* wo
   const dim3 grid((uint32_t)blocks), blk(256);
   {
   TimedScope ts(KC_GEMM_PACKED, st);
-  if (TM == 256 && TN == 128) {
+  if (np == 3) {
+    hipLaunchKernelGGL((k_gemm_packed_bf16x6_pipe<256, 128, 4, 2>), grid, dim3(512), 0, st, d);
+  } else if (TM == 256 && TN == 128) {
     if (stages != 2) hipLaunchKernelGGL((k_gemm_packed_bf16x3<256, 128, 1>), grid, blk, 0, st, d);
     else hipLaunchKernelGGL((k_gemm_packed_bf16x3<256, 128, 2>), grid, blk, 0, st, d);
   } else if (TN == 256 && TM == 128) {

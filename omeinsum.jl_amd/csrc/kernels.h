@@ -57,7 +57,7 @@ struct RangeScope {
 // Writes code into *status (a device-visible word) from a kernel on st.
 void launch_status_set(int* status, int code, hipStream_t st);
 
-enum Prec : int { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_FP64 = 2 };
+enum Prec : int { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_FP64 = 2, PREC_BF16X6 = 3 };
 enum GemmKernel : int { GEMM_AUTO = -1, GEMM_VALU_TILE = 0, GEMM_VALU_DIRECT = 1, GEMM_MFMA = 2 };
 
 void launch_axpby(int dt, const void* x, void* y, int64_t n, const Scal& sx, const Scal& sy,
@@ -257,12 +257,13 @@ int mfma_c32_plan_ksplit(const GemmDesc& d);
 int64_t mfma_c32_workspace(const GemmDesc& d, int ksplit);
 void launch_gemm_mfma_c32(GemmDesc d, int ksplit, hipStream_t st);
 
-// Packed BF16x3 GEMM (Float32, or ComplexF32 realified into one real GEMM).
-int64_t packed_bytes_a(int64_t M, int64_t N, int64_t K, int64_t batch);
-int64_t packed_bytes_b(int64_t M, int64_t N, int64_t K, int64_t batch);
-int64_t packed_tiles_mn(int64_t M, int64_t N);
+// Packed split-precision GEMM (Float32, or ComplexF32 realified into one real GEMM) on np
+// bf16 planes per operand: np = 2 (BF16x3, three products) or 3 (BF16x6, six products).
+int64_t packed_bytes_a(int64_t M, int64_t N, int64_t K, int64_t batch, int np);
+int64_t packed_bytes_b(int64_t M, int64_t N, int64_t K, int64_t batch, int np);
+int64_t packed_tiles_mn(int64_t M, int64_t N, int np);
 bool packed_plain_overwrite(const GemmDesc& g);
 void launch_gemm_packed(int dt, const GemmDesc& g, void* pa, void* pb, float* work, int ksplit,
-                        hipStream_t st);
+                        hipStream_t st, int np);
 
 }  // namespace ome

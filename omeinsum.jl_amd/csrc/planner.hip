@@ -226,7 +226,7 @@ static Status do_unary(ome_ctx* ctx, int dt, const void* x, const std::vector<in
 
 // BF16x3 on packed operands: Float32 and ComplexF32 products large enough that the one-pass
 // pack (split + tile-major relayout) is amortised by the GEMM.
-static Status run_gemm_packed(ome_ctx* ctx, int dt, const GemmDesc& d0, hipStream_t st) {
+static Status run_gemm_packed(ome_ctx* ctx, int dt, const GemmDesc& d0, hipStream_t st, int np) {
   const bool cplx = dt == C32;
   // The kernel's lanes run along m in its stores (32 consecutive m per half-wave).  When y is
   // contiguous along n instead, compute y^T = B^T A^T (swap the operands' roles) so that every
@@ -241,17 +241,17 @@ static Status run_gemm_packed(ome_ctx* ctx, int dt, const GemmDesc& d0, hipStrea
     std::swap(d.sym, d.syn);
   }
   const int64_t M = d.M, N = cplx ? 2 * d.N : d.N, K = cplx ? 2 * d.K : d.K;
-  const int64_t tiles = packed_tiles_mn(M, N) * d.B;
+  const int64_t tiles = packed_tiles_mn(M, N, np) * d.B;
   const int64_t ktiles = (K + 31) / 32;
   int ks = 1;
   while (tiles * ks < 256 && ktiles / (ks * 2) >= 8) ks *= 2;
-  Temp pa(ctx, (size_t)packed_bytes_a(M, N, K, d.B));
-  Temp pb(ctx, (size_t)packed_bytes_b(M, N, K, d.B));
+  Temp pa(ctx, (size_t)packed_bytes_a(M, N, K, d.B, np));
+  Temp pb(ctx, (size_t)packed_bytes_b(M, N, K, d.B, np));
   const bool via_work = ks > 1 || (cplx && !packed_plain_overwrite(d));
   Temp work(ctx, via_work ? (size_t)ks * d.B * M * N * 4 : 0);
   if (!pa.p || !pb.p || (via_work && !work.p))
     return Status::err(OME_ERR_OOM, "device pool exhausted (packed GEMM operands)");
-  launch_gemm_packed(dt, d, pa.p, pb.p, (float*)work.p, ks, st);
+  launch_gemm_packed(dt, d, pa.p, pb.p, (float*)work.p, ks, st, np);
   return Status::ok();
 }
 
@@ -301,7 +301,21 @@ Status run_gemm(ome_ctx* ctx, int dt, GemmDesc d, hipStream_t st, int precision)
       (kernel == GEMM_AUTO || kernel == GEMM_MFMA)) {
     const double flops = 2.0 * d.M * d.N * d.K * d.B * (dt == C32 ? 4 : 1);
     if (d.M >= 64 && d.N >= 64 && d.K >= 64 && flops >= 3.0e7)
-      return run_gemm_packed(ctx, dt, d, st);
+      return run_gemm_packed(ctx, dt, d, st, 2);
+  }
+  // six products on three planes: FP32-level error at a lower MFMA cost than FP32 operands.
+  // The pack reads and writes every operand element once more (about 10 bytes per element),
+  // which the cheaper products repay when M'N' / (M' + N') (output elements per operand
+  // element of one k) is a few hundred; smaller products run FP32 MFMA.  Split-K makes up for
+  // few 256 x 128 tiles over a long K.
+  if (precision == PREC_BF16X6 && (dt == F32 || dt == C32) &&
+      (kernel == GEMM_AUTO || kernel == GEMM_MFMA)) {
+    const double rm = (double)d.M, rn = (double)(dt == C32 ? 2 * d.N : d.N);
+    const double rk = (double)(dt == C32 ? 2 * d.K : d.K);
+    const double flops = 2.0 * rm * rn * rk * d.B;
+    if (std::max(rm, rn) >= 256 && std::min(rm, rn) >= 128 && rk >= 256 &&
+        rm * rn / (rm + rn) >= 200 && flops >= 2.0e8)
+      return run_gemm_packed(ctx, dt, d, st, 3);
   }
   if (kernel == GEMM_AUTO || kernel == GEMM_MFMA) {
     const bool fits = d.M * d.K * d.B < (1ll << 31) && d.K * d.N * d.B < (1ll << 31) &&

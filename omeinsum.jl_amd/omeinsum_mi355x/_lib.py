@@ -23,7 +23,7 @@ PLAN_OPTIONS = (OPT_PRECISION, OPT_COMPLEX_ALGO, OPT_ALLOW_LOOPS, OPT_DIRECT_GEM
 # kernel classes of ome_get_kernel_timing
 KC = {"gemm_packed": 0, "pack": 1, "gemm_mfma": 2, "gemm_valu": 3, "permute": 4, "walk": 5,
       "grouped": 6, "other": 7}
-PREC = {"fp32": 0, "bf16x3": 1, "fp64": 2}
+PREC = {"fp32": 0, "bf16x3": 1, "fp64": 2, "bf16x6": 3}
 
 c_int, c_i32, c_i64, c_void_p, c_size_t, c_double, c_char = (
     ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_size_t,

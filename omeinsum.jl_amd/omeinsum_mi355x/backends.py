@@ -8,7 +8,9 @@
 * ``MI355XBackend`` — contraction trees are compiled once into a device plan: intermediates
   in one arena laid out by a static liveness plan, the step sequence recorded into a
   hipGraph and replayed, sliced trees summed in FP64 on the device.  Float32/ComplexF32
-  GEMM-shaped steps use ``precision`` ("fp32", "bf16x3" or "fp64").
+  GEMM-shaped steps use ``precision``: "fp32" (FP32 MFMA), "bf16x3" (three products of bf16
+  hi/lo planes), "bf16x6" (six products of three bf16 planes: FP32-level error; large
+  products only, the rest at FP32) or "fp64" (widened).
 """
 from . import _lib
 from . import ein_ops as E

@@ -60,7 +60,8 @@ def _mat(t):
 
 @pytest.mark.parametrize("dtype,prec", [(torch.float32, "fp32"), (torch.float32, "bf16x3"),
                                         (torch.float32, "fp64"), (torch.complex64, "fp32"),
-                                        (torch.complex64, "bf16x3")])
+                                        (torch.complex64, "bf16x3"), (torch.float32, "bf16x6"),
+                                        (torch.complex64, "bf16x6")])
 def test_c4_abcd_cdef_64_forward_backward(dtype, prec):
     o = om()
     a, b, dy = _c4_inputs(dtype, 4100 + 7 * (dtype == torch.complex64))
@@ -155,12 +156,12 @@ def _network_100():
 # intermediates in ComplexF32 (9e-7 here) is the floor of every ComplexF32 evaluation.  fp32
 # adds the FP32 accumulation of each product.  bf16x3 adds the split's ~2^-16.5 relative
 # representation error of every operand of the split products, which the network's
-# conditioning amplifies (2.1e-5 measured here; the bound allows 2.5x).  ComplexF64 inputs
-# stay in FP64.
-C2_BOUNDS = {"fp32": 1e-5, "bf16x3": 5e-5, "fp64": 2e-6, "c128": 1e-12}
+# conditioning amplifies (2.1e-5 measured here; the bound allows 2.5x).  bf16x6 keeps ~24 bits
+# of every operand: the fp32 bound.  ComplexF64 inputs stay in FP64.
+C2_BOUNDS = {"fp32": 1e-5, "bf16x3": 5e-5, "fp64": 2e-6, "c128": 1e-12, "bf16x6": 1e-5}
 
 
-@pytest.mark.parametrize("kind", ["fp32", "bf16x3", "fp64", "c128"])
+@pytest.mark.parametrize("kind", ["fp32", "bf16x3", "fp64", "c128", "bf16x6"])
 def test_c2_network_100_scalar(kind):
     o = om()
     tree, sd, xs, ref = _network_100()
@@ -199,11 +200,12 @@ def _stream(sc_budget=None):
 # ComplexF32 storage of the intermediates alone costs up to 2.6e-6 on these networks (request
 # 28, 134 vertices: every step in FP64, rounded to ComplexF32); products sum at most 8192 real
 # terms per FP32 accumulator.  BF16x3: the split's representation error, amplified by each
-# network's conditioning: up to 1.3e-4 measured (request 14), within 3e-4.
-C5_BOUNDS = {"fp32": 1e-5, "bf16x3": 3e-4}
+# network's conditioning: up to 1.3e-4 measured (request 14), within 3e-4.  BF16x6: the fp32
+# bound (its large products keep ~24 bits of every operand and accumulate in FP32).
+C5_BOUNDS = {"fp32": 1e-5, "bf16x3": 3e-4, "bf16x6": 1e-5}
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16x6"])
 def test_c5_request_stream_alone_and_mixed(prec):
     o = om()
     from omeinsum_mi355x.server import Server

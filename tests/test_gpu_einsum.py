@@ -411,3 +411,54 @@ def test_packed_bf16x3(dtype, spec, scal):
     if not isinstance(sx, bool):
         ref = sx * ref + sy * y0.astype(W)
     assert_close(got, ref, dtype, 16 * 520, spec, precision="bf16x3", depth=520)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
+@pytest.mark.parametrize("spec", ["ij,jk->ik", "ji,kj->ik", "ijb,jkb->ikb", "ajb,bkj->bka", "ij,jk->ki"])
+@pytest.mark.parametrize("scal", [(True, False), (1.5 - 0.5j, 0.25)])
+def test_packed_bf16x6(dtype, spec, scal):
+    # the six-product packed path: depth 256 (512 for complex), where a kernel that dropped one
+    # of its three 2^-16 products would fail the bf16x6 bound
+    o = om()
+    ixs, iy = parse(spec)
+    sd = {"i": 1200, "j": 256, "k": 1000, "a": 300, "b": 2}
+    r = rng(1418)
+    xs = [rand(r, [sd[l] for l in ix], dtype) for ix in ixs]
+    sx, sy = scal
+    if dtype == np.float32 and not isinstance(sx, bool):
+        sx = 1.5
+    y0 = rand(r, [sd[l] for l in iy], dtype)
+    o.set_einsum_backend(o.MI355XBackend(precision="bf16x6"))
+    try:
+        got = run(ixs, iy, xs, sd, y0=y0, sx=sx, sy=sy)
+    finally:
+        o.set_einsum_backend(o.DefaultBackend())
+    W = wide(dtype)
+    ref = db.einsum(ixs, iy, [x.astype(W) for x in xs], sd)
+    if not isinstance(sx, bool):
+        ref = sx * ref + sy * y0.astype(W)
+    assert_close(got, ref, dtype, 256, spec, precision="bf16x6", depth=256)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
+@pytest.mark.parametrize("sd", [{"i": 1201, "j": 260, "k": 999},      # ragged tiles and k tiles
+                                {"i": 768, "j": 8192, "k": 384}])     # few tiles: split-K
+def test_packed_bf16x6_edges(dtype, sd):
+    o = om()
+    ixs, iy = parse("ij,jk->ik")
+    r = rng(1419)
+    xs = [rand(r, [sd[l] for l in ix], dtype) for ix in ixs]
+    o.set_einsum_backend(o.MI355XBackend(precision="bf16x6"))
+    ctx = o._lib.context()
+    ctx.set_option(o._lib.OPT_KERNEL_TIMING, 1)
+    ctx.kernel_timing("gemm_packed")
+    try:
+        got = run(ixs, iy, xs, sd)
+        _, n_packed = ctx.kernel_timing("gemm_packed")
+    finally:
+        ctx.set_option(o._lib.OPT_KERNEL_TIMING, 0)
+        o.set_einsum_backend(o.DefaultBackend())
+    assert n_packed == 1, "the product did not take the packed six-product path"
+    W = wide(dtype)
+    ref = xs[0].astype(W) @ xs[1].astype(W)
+    assert_close(got, ref, dtype, sd["j"], "bf16x6 %s" % sd, precision="bf16x6", depth=sd["j"])

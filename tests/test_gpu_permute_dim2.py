@@ -60,3 +60,35 @@ def test_dim2_permutes(dtype, rank, seed):
         o.tensorpermute(yt, t, perm, 3, -2)
         torch.cuda.synchronize()
         assert np.array_equal(o.to_numpy(yt), 3 * ref - 2 * y0)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.complex64, np.complex128, np.int64,
+                                   np.bool_])
+@pytest.mark.parametrize("seed", range(6))
+def test_power_of_two_permutes(dtype, seed):
+    # extents 1, 2, 4, 8 and 64 (every one a power of two: the bit-permutation tile), 2^8 to
+    # 2^20 elements, random permutations; exact, with and without scaling
+    o = om()
+    r = rng(4200 + seed)
+    while True:
+        rank = int(r.integers(2, 14))
+        shape = [int(r.choice([1, 2, 2, 2, 4, 8, 64])) for _ in range(rank)]
+        n = int(np.prod(shape))
+        if 256 <= n <= (1 << 20):
+            break
+    perm = [int(p) for p in r.permutation(rank)]
+    x = values(r, shape, dtype)
+    ref = np.transpose(x, perm)
+    t = torch.from_numpy(x).cuda()
+    y = torch.empty(ref.shape, dtype=t.dtype, device="cuda")
+    if dtype == np.bool_:
+        t, y = o.TropicalAndOr(t), o.TropicalAndOr(y)
+    o.tensorpermute(y, t, perm)
+    torch.cuda.synchronize()
+    assert np.array_equal(o.to_numpy(y), ref), (shape, perm)
+    if dtype not in (np.bool_,):
+        y0 = values(r, ref.shape, dtype)
+        yt = torch.from_numpy(np.ascontiguousarray(y0)).cuda()
+        o.tensorpermute(yt, t, perm, 3, -2)
+        torch.cuda.synchronize()
+        assert np.array_equal(o.to_numpy(yt), 3 * ref - 2 * y0), (shape, perm)

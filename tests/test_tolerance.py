@@ -4,7 +4,11 @@ A numpy model of the 3-term BF16 split product (hi = RNE_bf16(x), lo = RNE_bf16(
 C = ah*bh + ah*bl + al*bh summed exactly and rounded once to Float32) passes the bf16x3 bound
 on the shape and seed of the GPU test of the 256 x 256-tile kernel, while the same model with
 one correction product dropped fails it, and the model itself fails the fp32 bound: the bound
-separates the arithmetic the device is meant to run from a plausible mistake in it."""
+separates the arithmetic the device is meant to run from a plausible mistake in it.
+
+The same for the 3-plane split (hi, mid, lo; the six products a_p*b_q with p + q <= 2): summed
+in FP32 over K at depth 256 it passes the bf16x6 bound, and each model that drops one of the
+three 2^-16 products (a_lo*b_hi, a_mid*b_mid, a_hi*b_lo) fails it."""
 import numpy as np
 import pytest
 
@@ -78,3 +82,58 @@ def test_fp32_rounding_passes_the_fp32_bound(operands):
     bad = acc - np.outer(a[rows, 519], b[519]).astype(np.float32) * np.float32(1e-3)
     with pytest.raises(AssertionError, match="norm-wise"):
         assert_close(bad, ref[rows], np.float32, 16 * 520, "perturbed", precision="fp32", depth=520)
+
+
+def split3(x):
+    x = np.asarray(x, np.float32)
+    hi = bf16_rne(x)
+    r1 = (x - hi).astype(np.float32)
+    mid = bf16_rne(r1)
+    lo = bf16_rne((r1 - mid).astype(np.float32))
+    return [v.astype(np.float64) for v in (hi, mid, lo)]
+
+
+SIX = [(2, 0), (1, 1), (0, 2), (1, 0), (0, 1), (0, 0)]
+
+
+@pytest.fixture(scope="module")
+def operands256():
+    # depth 256: where the GPU tests of the six-product kernel pin it
+    r = rng(1416)
+    a = rand(r, (256, 256), np.float32)
+    b = rand(r, (256, 1200), np.float32)
+    return a, b, a.astype(np.float64) @ b.astype(np.float64)
+
+
+def six_product_fp32(a, b, products):
+    """C accumulated in FP32 over K in order, the listed products of each k added in turn."""
+    A, B = split3(a), split3(b)
+    acc = np.zeros((a.shape[0], b.shape[1]), np.float32)
+    for k in range(a.shape[1]):
+        for p, q in products:
+            acc += np.outer(A[p][:, k], B[q][k]).astype(np.float32)
+    return acc
+
+
+def test_three_plane_split_reconstructs_to_fp32():
+    r = rng(1417)
+    x = rand(r, (4096,), np.float32)
+    hi, mid, lo = split3(x)
+    # hi + mid + lo is within 2^-24 |x| of x (Float32 has 24 bits)
+    assert np.all(np.abs(hi + mid + lo - x.astype(np.float64)) <= 2.0 ** -24 * np.abs(x))
+    assert np.all(np.abs(mid) <= 2.0 ** -8 * np.abs(x)) and np.all(np.abs(lo) <= 2.0 ** -16 * np.abs(x))
+
+
+def test_six_product_split_passes_the_bf16x6_bound(operands256):
+    a, b, ref = operands256
+    c = six_product_fp32(a, b, SIX)
+    assert_close(c, ref, np.float32, 256, "6-product split", precision="bf16x6", depth=256)
+
+
+@pytest.mark.parametrize("dropped", [(2, 0), (1, 1), (0, 2)])
+def test_five_product_split_fails_the_bf16x6_bound(operands256, dropped):
+    a, b, ref = operands256
+    c = six_product_fp32(a, b, [pq for pq in SIX if pq != dropped])
+    assert rel_err(c, ref) > 1.5 * norm_bound("bf16x6", 256)
+    with pytest.raises(AssertionError, match="norm-wise"):
+        assert_close(c, ref, np.float32, 256, "5-product split", precision="bf16x6", depth=256)
