@@ -355,6 +355,12 @@ int ome_server_get_stats(ome_server* srv, ome_server_stats* stats);
  * shared by all of them, then each its other steps.  Each request computes exactly what it
  * computes alone. */
 int ome_server_set_batching(ome_server* srv, int on);
+/* Admission hold (no reference counterpart; for callers that submit a burst of requests to
+ * start together, and for tests of batching): while on, submitted requests queue without
+ * being admitted; turning it off admits every queued request that fits in one round.  A
+ * request submitted under hold completes only after the hold is released, so release it
+ * before waiting on such a request.  Destroying the server admits and drains the queue. */
+int ome_server_hold(ome_server* srv, int on);
 /* Request status word (no reference counterpart; for tests of failure handling): every
  * execution slot owns a device-visible status word, cleared when a request is admitted to it
  * and read when the request retires; a non-zero word fails that request alone

@@ -88,6 +88,7 @@ struct ome_server {
   ome_server_stats stats{};
   // the shared launch of fused subtrees across requests
   bool batching = true;
+  bool hold = false;                         // admission held (ome_server_hold)
   hipStream_t bst = nullptr;                 // batch stream
   hipEvent_t round_admit = nullptr, batch_done = nullptr, copy_ev[2] = {nullptr, nullptr};
   char* job_host[2] = {nullptr, nullptr};    // pinned staging of the job table, two buffers
@@ -259,8 +260,10 @@ void scheduler(ome_server* s) {
       progress = true;
     }
     if (progress) s->cv_done.notify_all();
-    // 2. admit queued requests that fit, in arrival order: claim a slot and memory
-    for (auto it = s->queue.begin(); it != s->queue.end();) {
+    // 2. admit queued requests that fit, in arrival order: claim a slot and memory (none
+    // while admission is held, unless the server is stopping)
+    const bool admit = !s->hold || s->stop;
+    for (auto it = s->queue.begin(); admit && it != s->queue.end();) {
       Slot* free_slot = nullptr;
       for (auto& sl : s->slots)
         if (!sl.req) { free_slot = &sl; break; }
@@ -341,7 +344,7 @@ void scheduler(ome_server* s) {
     if (s->stop && s->queue.empty() && s->active == 0) break;
     if (!progress) {
       if (s->active == 0) {
-        s->cv_work.wait(lk, [s] { return s->stop || !s->queue.empty(); });
+        s->cv_work.wait(lk, [s] { return s->stop || (!s->queue.empty() && !s->hold); });
       } else {
         lk.unlock();
         std::this_thread::sleep_for(std::chrono::microseconds(20));
@@ -537,6 +540,16 @@ int ome_server_set_batching(ome_server* s, int on) {
   if (!s) return OME_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> lk(s->mu);
   s->batching = on != 0;
+  return OME_OK;
+}
+
+int ome_server_hold(ome_server* s, int on) {
+  if (!s) return OME_ERR_INVALID_ARG;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->hold = on != 0;
+  }
+  s->cv_work.notify_all();
   return OME_OK;
 }
 

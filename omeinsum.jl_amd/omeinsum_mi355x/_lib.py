@@ -110,6 +110,7 @@ SIGNATURES = {
     "ome_server_set_request_limit": (c_int, [c_void_p, c_size_t]),
     "ome_server_get_stats": (c_int, [c_void_p, P(ServerStats)]),
     "ome_server_set_batching": (c_int, [c_void_p, c_int]),
+    "ome_server_hold": (c_int, [c_void_p, c_int]),
     "ome_server_inject_status": (c_int, [c_void_p, c_int]),
 }
 

@@ -85,6 +85,11 @@ class Server:
             entry = self._live.pop(ticket, None)
         return entry[2], lat.value
 
+    def hold(self, on=True):
+        """Hold admission (on) or release it (off): requests submitted while held queue and
+        are admitted together, in one round, on release.  Release before waiting on them."""
+        _lib.check(self.ctx.lib.ome_server_hold(self.h, 1 if on else 0), self.ctx.h)
+
     def stats(self):
         """Counters of the loop: submitted, completed, sliced (admitted as a sliced twin),
         blocks_reused (lent the block the same plan used last), kept_bytes, batched_launches

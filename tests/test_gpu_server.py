@@ -197,7 +197,10 @@ def test_cross_request_batching_matches_unbatched():
         with Server(hbm_budget=1 << 30, max_active=8, batching=batching) as srv:
             plans = [srv.plan(tree, dxs, sd) for tree, sd, xs, dxs in reqs]
             for _ in range(2):  # the second pass replays graphs recorded without fused steps
+                # submitted under hold: admitted in rounds of up to 8 whatever the timing
+                srv.hold(True)
                 tickets = [srv.submit(p, r[3]) for p, r in zip(plans, reqs)]
+                srv.hold(False)
                 outs[batching] = [complex(om().to_numpy(srv.wait(t)[0])[()]) for t in tickets]
             st = srv.stats()
         if batching:
