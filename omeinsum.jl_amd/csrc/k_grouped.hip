@@ -287,10 +287,12 @@ __global__ __launch_bounds__(256) void k_grouped_f32(GGDesc g, const GGProb* __r
       __syncthreads();
       if (k0 + BK < K) load(k0 + BK);
       // lanes 0-31 take k = kb + s, lanes 32-63 k = kb + 4 + s in the s-th MFMA of a group;
-      // a wave whose whole block lies outside the problem skips its MFMAs
+      // a wave whose whole block lies outside the problem skips its MFMAs, and the 8-deep
+      // groups past K (the zero padding of the last step) are skipped
       if (m0 + wm < M && n0 + wn < N) {
 #pragma unroll
         for (int kb = 0; kb < BK; kb += 8) {
+          if (k0 + kb >= K) break;
           const int kk = kb + 4 * (lane >> 5);
           float4 af[NB], bf[MB];
 #pragma unroll
@@ -471,6 +473,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 &&
       if (m0 + wm < M && n0 + wn < N) {
 #pragma unroll
         for (int s2 = 0; s2 < BK / 16; ++s2) {
+          if (k0 + 16 * s2 >= K) break;  // a 16-deep sub-step wholly in the padding
           // lane l holds row l % 32, k = 16*s2 + 8*(l / 32) + 0..7
           const int kk = 16 * s2 + 8 * (lane >> 5);
           bf16x8_t ah[NB], al[NB], bh[MB], bl[MB];
@@ -591,6 +594,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gg_run_per_
   for (int q = 0; q < 16; ++q) acc[q] = 0.f;
   for (;;) {
     const bool last = lk0 + BK >= lK;  // the step in registers ends its tile
+    const uint32_t kv = lK > lk0 ? lK - lk0 : 0;  // k of this step inside the problem
     const bool more = !last || ntile < xend;
     // the step in registers -> LDS (BF16x3: split into hi and lo halves)
     if constexpr (X3) {
@@ -630,6 +634,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gg_run_per_
       if constexpr (X3) {
 #pragma unroll
         for (int s2 = 0; s2 < BK / 16; ++s2) {
+          if (16u * s2 >= kv) break;
           const int kk = 16 * s2 + 8 * (lane >> 5);
           const bf16x8_t ah = *(const bf16x8_t*)&sB[0][wn + (lane & 31)][kk];
           const bf16x8_t al = *(const bf16x8_t*)&sB[1][wn + (lane & 31)][kk];
@@ -642,6 +647,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(gg_run_per_
       } else {
 #pragma unroll
         for (int kb = 0; kb < BK; kb += 8) {
+          if ((uint32_t)kb >= kv) break;
           const int kk = kb + 4 * (lane >> 5);
           const float4 af = *(const float4*)&sB[0][wn + (lane & 31)][kk];
           const float4 bf = *(const float4*)&sA[0][wm + (lane & 31)][kk];
