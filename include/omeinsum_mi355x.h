@@ -104,8 +104,8 @@ typedef enum ome_option {
                                   6 BF16x3 one wave per 32x32 tile, 7 BF16x3 64x64 with K
                                   step 64, 8 64x64 persistent and pipelined across tiles,
                                   9 BF16x3 64x64 persistent and pipelined (8 and 9 need
-                                  buffer resources and otherwise run as 1 and 3; Float32
-                                  only; for tests) */
+                                  buffer resources and otherwise run as 1 and 3),
+                                  10 BF16x6 64x64 (Float32 only; for tests) */
   OME_OPT_LAST_GROUPED_CHOICE = 9, /* read-only: 2*variant + persistent of the last grouped
                                       GEMM launch (the autotuner's pick) */
   OME_OPT_TREE_DEFER_MEMORY = 10,  /* 1: ome_tree_create sizes the plan's device memory but

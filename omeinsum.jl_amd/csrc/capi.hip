@@ -369,7 +369,7 @@ int ome_set_option(ome_ctx* ctx, int key, int64_t v) {
         return fail(ctx, OME_ERR_UNSUPPORTED, "roctx library (librocprofiler-sdk-roctx) not found");
       break;
     case OME_OPT_GROUPED_VARIANT:
-      if (v < -1 || v >= GG_NVARIANTS) return fail(ctx, OME_ERR_INVALID_ARG, "grouped variant must be -1..9");
+      if (v < -1 || v >= GG_NVARIANTS) return fail(ctx, OME_ERR_INVALID_ARG, "grouped variant must be -1..10");
       ctx->opt.grouped_variant = (int)v; break;
     default: return fail(ctx, OME_ERR_INVALID_ARG, "unknown option");
   }

@@ -259,6 +259,12 @@ static std::vector<GGCand> gg_candidates(int dt, int precision, double mean_mn) 
     c.push_back({GG_BF16X3_64_BK64, 0});
     c.push_back({GG_BF16X3_RUN, 0});
     if (mean_mn >= 96) c.push_back({GG_BF16X3_128, 0});
+  } else if (dt == F32 && precision == PREC_BF16X6) {
+    // FP32-accurate: the six-product split against the FP32 kernels, the faster one kept
+    c.push_back({GG_BF16X6_64, 0});
+    c.push_back({GG_BF16X6_64, 1});
+    c.push_back({GG_F32_64_BK32, 0});
+    c.push_back({GG_F32_RUN, 0});
   } else if (dt == F32) {
     c.push_back({GG_F32_W32, 1});
     c.push_back({GG_F32_W32, 0});
@@ -339,8 +345,8 @@ Status run_grouped_gemm(ome_ctx* ctx, int dt, const std::vector<GGProblem>& ps, 
   }
   if (w <= 0) w = 1;
   smn = (sm + sn) / (2 * w);
-  // Float32 follows the context's precision (FP32 or BF16x3 products); FP64 precision has
-  // no grouped form and runs as FP32
+  // Float32 follows the context's precision (FP32, BF16x3 or BF16x6 products); FP64
+  // precision has no grouped form and runs as FP32
   const int prec = ctx->opt.precision;
   const auto cands = gg_candidates(dt, prec, smn);
   int pick = -1;

@@ -360,10 +360,12 @@ __device__ __forceinline__ void gg_split(float x, uint16_t& hi, uint16_t& lo) {
 // check returns zero past the slice, and C is written through one, whose range check drops
 // the stores past it; only a row past M (A, C) or a k past K (B) needs an explicit check.
 // BK: k per step (32, or 64 for half the round trips of a short tile at the cost of LDS).
-template <int TM, int TN, bool BUF, int BK = 32>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 && BK == 32 ? (BUF ? 6 : 4) : 1, TM == 64 && BK == 32 ? (BUF ? 6 : 4) : 8))) void k_grouped_bf16x3(GGDesc g, const GGProb* __restrict__ probs,
-                                                        const int64_t* __restrict__ first,
-                                                        const int32_t* __restrict__ map) {
+// NP: bf16 planes per operand: 2 (BF16x3: three products) or 3 (BF16x6: hi, mid, lo and the
+// six products a_p*b_q with p + q <= 2, smallest first; FP32-level error).
+template <int TM, int TN, bool BUF, int BK, int NP>
+__device__ __forceinline__ void grouped_split_body(const GGDesc& g, const GGProb* __restrict__ probs,
+                                                   const int64_t* __restrict__ first,
+                                                   const int32_t* __restrict__ map) {
   constexpr int LDK = BK + 8;  // row pitch 80 or 144 bytes: 8 rows hit 8 distinct bank groups
   constexpr int MB = TM / 64, NB = TN / 64;
   constexpr int NBL = TN * BK / 1024;
@@ -372,8 +374,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 &&
   // row pitch puts eight consecutive rows in distinct bank groups
   constexpr int EPT = TM * BK / 256, KQ = BK / EPT;
   static_assert(TM * KQ == 256 && EPT % 8 == 0 && EPT <= 32, "A staging shape");
-  __shared__ __attribute__((aligned(16))) uint16_t sA[2][TM][LDK];  // [hi/lo][m][k]
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][TN][LDK];  // [hi/lo][n][k]
+  __shared__ __attribute__((aligned(16))) uint16_t sA[NP][TM][LDK];  // [plane][m][k]
+  __shared__ __attribute__((aligned(16))) uint16_t sB[NP][TN][LDK];  // [plane][n][k]
   const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = (w & 1) * (TM / 2), wn = (w >> 1) * (TN / 2);
   const float alpha = (float)g.alpha.re, beta = (float)g.beta.re;
@@ -450,23 +452,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 &&
       }
 #pragma unroll
       for (int c = 0; c < EPT / 8; ++c) {
-        uint4 ph, pl;
-        split2_bf16(ra[8 * c + 0], ra[8 * c + 1], ph.x, pl.x);
-        split2_bf16(ra[8 * c + 2], ra[8 * c + 3], ph.y, pl.y);
-        split2_bf16(ra[8 * c + 4], ra[8 * c + 5], ph.z, pl.z);
-        split2_bf16(ra[8 * c + 6], ra[8 * c + 7], ph.w, pl.w);
-        *(uint4*)&sA[0][am][ak + 8 * c] = ph;
-        *(uint4*)&sA[1][am][ak + 8 * c] = pl;
+        uint4 p0, p1, p2;
+        if constexpr (NP == 2) {
+          split2_bf16(ra[8 * c + 0], ra[8 * c + 1], p0.x, p1.x);
+          split2_bf16(ra[8 * c + 2], ra[8 * c + 3], p0.y, p1.y);
+          split2_bf16(ra[8 * c + 4], ra[8 * c + 5], p0.z, p1.z);
+          split2_bf16(ra[8 * c + 6], ra[8 * c + 7], p0.w, p1.w);
+        } else {
+          split3_bf16(ra[8 * c + 0], ra[8 * c + 1], p0.x, p1.x, p2.x);
+          split3_bf16(ra[8 * c + 2], ra[8 * c + 3], p0.y, p1.y, p2.y);
+          split3_bf16(ra[8 * c + 4], ra[8 * c + 5], p0.z, p1.z, p2.z);
+          split3_bf16(ra[8 * c + 6], ra[8 * c + 7], p0.w, p1.w, p2.w);
+          *(uint4*)&sA[NP - 1][am][ak + 8 * c] = p2;
+        }
+        *(uint4*)&sA[0][am][ak + 8 * c] = p0;
+        *(uint4*)&sA[1][am][ak + 8 * c] = p1;
       }
 #pragma unroll
       for (int q = 0; q < NBL; ++q) {
         const int e = t + 256 * q;
         const int k = (e % (BK / 4)) * 4, n = e / (BK / 4);
-        uint2 ph, pl;
-        split2_bf16(rb[q].x, rb[q].y, ph.x, pl.x);
-        split2_bf16(rb[q].z, rb[q].w, ph.y, pl.y);
-        *(uint2*)&sB[0][n][k] = ph;
-        *(uint2*)&sB[1][n][k] = pl;
+        uint2 p0, p1, p2;
+        if constexpr (NP == 2) {
+          split2_bf16(rb[q].x, rb[q].y, p0.x, p1.x);
+          split2_bf16(rb[q].z, rb[q].w, p0.y, p1.y);
+        } else {
+          split3_bf16(rb[q].x, rb[q].y, p0.x, p1.x, p2.x);
+          split3_bf16(rb[q].z, rb[q].w, p0.y, p1.y, p2.y);
+          *(uint2*)&sB[NP - 1][n][k] = p2;
+        }
+        *(uint2*)&sB[0][n][k] = p0;
+        *(uint2*)&sB[1][n][k] = p1;
       }
       __syncthreads();
       if (k0 + BK < K) load(k0 + BK);
@@ -474,27 +490,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 &&
 #pragma unroll
         for (int s2 = 0; s2 < BK / 16; ++s2) {
           if (k0 + 16 * s2 >= K) break;  // a 16-deep sub-step wholly in the padding
-          // lane l holds row l % 32, k = 16*s2 + 8*(l / 32) + 0..7
+          // lane l holds row l % 32, k = 16*s2 + 8*(l / 32) + 0..7; fn: plane p of the B
+          // tile (rows n, the MFMA's first operand), fm: plane p of the A tile (rows m)
           const int kk = 16 * s2 + 8 * (lane >> 5);
-          bf16x8_t ah[NB], al[NB], bh[MB], bl[MB];
+          bf16x8_t fn[NP][NB], fm[NP][MB];
 #pragma unroll
-          for (int nb = 0; nb < NB; ++nb) {
-            ah[nb] = *(const bf16x8_t*)&sB[0][wn + nb * 32 + (lane & 31)][kk];
-            al[nb] = *(const bf16x8_t*)&sB[1][wn + nb * 32 + (lane & 31)][kk];
+          for (int p = 0; p < NP; ++p) {
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+              fn[p][nb] = *(const bf16x8_t*)&sB[p][wn + nb * 32 + (lane & 31)][kk];
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb)
+              fm[p][mb] = *(const bf16x8_t*)&sA[p][wm + mb * 32 + (lane & 31)][kk];
           }
+          // products a_p * b_q, smallest first: (1,0) (0,1) (0,0) for two planes;
+          // (2,0) (1,1) (0,2) (1,0) (0,1) (0,0) for three
+          constexpr int NQ = NP == 2 ? 3 : 6;
+          constexpr int PN2[3] = {1, 0, 0}, PM2[3] = {0, 1, 0};
+          constexpr int PN3[6] = {2, 1, 0, 1, 0, 0}, PM3[6] = {0, 1, 2, 0, 1, 0};
 #pragma unroll
-          for (int mb = 0; mb < MB; ++mb) {
-            bh[mb] = *(const bf16x8_t*)&sA[0][wm + mb * 32 + (lane & 31)][kk];
-            bl[mb] = *(const bf16x8_t*)&sA[1][wm + mb * 32 + (lane & 31)][kk];
-          }
+          for (int q = 0; q < NQ; ++q)
 #pragma unroll
-          for (int nb = 0; nb < NB; ++nb)
+            for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb) {
-              acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[nb], bh[mb], acc[nb][mb], 0, 0, 0);
-              acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[nb], bl[mb], acc[nb][mb], 0, 0, 0);
-              acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[nb], bh[mb], acc[nb][mb], 0, 0, 0);
-            }
+              for (int mb = 0; mb < MB; ++mb)
+                acc[nb][mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    fn[NP == 2 ? PN2[q % 3] : PN3[q]][nb], fm[NP == 2 ? PM2[q % 3] : PM3[q]][mb],
+                    acc[nb][mb], 0, 0, 0);
         }
       }
       __syncthreads();
@@ -518,6 +540,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 &&
           }
     }
   }
+}
+
+template <int TM, int TN, bool BUF, int BK = 32>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM == 64 && BK == 32 ? (BUF ? 6 : 4) : 1, TM == 64 && BK == 32 ? (BUF ? 6 : 4) : 8))) void k_grouped_bf16x3(GGDesc g, const GGProb* __restrict__ probs,
+                                                        const int64_t* __restrict__ first,
+                                                        const int32_t* __restrict__ map) {
+  grouped_split_body<TM, TN, BUF, BK, 2>(g, probs, first, map);
+}
+
+// The six-product form, 64 x 64 tiles, K step 32 (30 KiB of LDS).
+template <bool BUF>
+__global__ __launch_bounds__(256) void k_grouped_bf16x6(GGDesc g, const GGProb* __restrict__ probs,
+                                                        const int64_t* __restrict__ first,
+                                                        const int32_t* __restrict__ map) {
+  grouped_split_body<64, 64, BUF, 32, 3>(g, probs, first, map);
 }
 
 // ---- grouped GEMM, persistent and pipelined across tiles ----------------------------------
@@ -966,6 +1003,10 @@ void launch_grouped_gemm(int dt, int variant, const GGDesc& g, int grid, hipStre
       hipLaunchKernelGGL((k_grouped_f32_wave<4>), dim3(grid), dim3(256), 0, st, g);
     else if (variant == GG_BF16X3_W32)
       hipLaunchKernelGGL((k_grouped_bf16x3_wave<2>), dim3(grid), dim3(256), 0, st, g);
+    else if (variant == GG_BF16X6_64 && g.offsets32)
+      hipLaunchKernelGGL((k_grouped_bf16x6<true>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
+    else if (variant == GG_BF16X6_64)
+      hipLaunchKernelGGL((k_grouped_bf16x6<false>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     else if (variant == GG_BF16X3_64_BK64 && g.offsets32)
       hipLaunchKernelGGL((k_grouped_bf16x3<64, 64, true, 64>), dim3(grid), dim3(256), 0, st, g, g.probs, g.first, g.map);
     else if (variant == GG_BF16X3_64_BK64)

@@ -193,7 +193,8 @@ struct GGDesc {
 enum GGVariant : int {
   GG_F32_64 = 0, GG_F32_64_BK32 = 1, GG_F32_128 = 2, GG_BF16X3_64 = 3, GG_BF16X3_128 = 4,
   GG_F32_W32 = 5, GG_BF16X3_W32 = 6, GG_BF16X3_64_BK64 = 7, GG_F32_RUN = 8, GG_BF16X3_RUN = 9,
-  GG_NVARIANTS = 10
+  GG_BF16X6_64 = 10,
+  GG_NVARIANTS = 11
 };
 // Workgroups per CU of the run variants (persistent, one per SIMD wave slot they fit: 20 KiB
 // of LDS each; BF16x3 in 72 VGPRs, FP32 in 80).

@@ -99,7 +99,7 @@ def uneven():
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16x6"])
 def test_c3_grouped_gemms_full_size(uneven, prec):
     from omeinsum_mi355x import _lib
     gemms, _, _, refs = uneven

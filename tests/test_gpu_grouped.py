@@ -26,6 +26,11 @@ def om():
 
 
 SPLIT = (3, 4, 6, 7, 9)  # the tile variants with BF16x3 products
+SPLIT6 = (10,)           # the tile variant with BF16x6 products
+
+
+def variant_precision(variant):
+    return "bf16x3" if variant in SPLIT else "bf16x6" if variant in SPLIT6 else "fp32"
 
 
 def problems(seed, n, dtype, dims=range(1, 201), batches=range(1, 5)):
@@ -74,7 +79,7 @@ def check(ps, got, alpha, beta, dtype, precision="fp32"):
 
 
 @pytest.mark.parametrize("buffers", [1, 0])
-@pytest.mark.parametrize("variant", [-1, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [-1, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 def test_grouped_gemm_f32_variants(variant, buffers):
     # buffers 0 forces the pointer form of the kernels that otherwise read through buffer
     # resources (the form problems above 32-bit offsets take)
@@ -84,7 +89,7 @@ def test_grouped_gemm_f32_variants(variant, buffers):
     ctx.set_option(_lib.OPT_GROUPED_BUFFERS, buffers)
     try:
         ps = problems(3100 + variant, 120, np.float32)
-        prec = "bf16x3" if variant in SPLIT else "fp32"
+        prec = variant_precision(variant)
         got = run(ps, True, False)
         check(ps, got, 1.0, 0.0, np.float32, prec)
         got = run(ps, 1.5, -0.5)
@@ -115,11 +120,11 @@ def test_grouped_gemm_runs_cross_tiles_and_problems(variant):
         ctx.set_option(_lib.OPT_GROUPED_VARIANT, -1)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "bf16x6"])
 def test_grouped_gemm_c3_shapes_autotuned_twice(prec):
     # dims in {8, 16, ..., 256} and batch in 1..16 as in the uneven-batch workload; the second
     # call reuses the tuned choice and the cached descriptors; the context's precision picks
-    # the FP32 or the BF16x3 kernels
+    # the FP32, the BF16x3 or the FP32-accurate kernels (BF16x6 or FP32, the faster)
     from omeinsum_mi355x import _lib
     ctx = _lib.context()
     ctx.set_option(_lib.OPT_PRECISION, _lib.PREC[prec])
@@ -130,6 +135,8 @@ def test_grouped_gemm_c3_shapes_autotuned_twice(prec):
             check(ps, got, 1.0, 0.0, np.float32, prec)
             variant = ctx.get_option(_lib.OPT_LAST_GROUPED_CHOICE) // 2
             assert (variant in SPLIT) == (prec == "bf16x3"), variant
+            if prec == "bf16x6":
+                assert variant in SPLIT6 + (1, 8), variant
     finally:
         ctx.set_option(_lib.OPT_PRECISION, 0)
 
