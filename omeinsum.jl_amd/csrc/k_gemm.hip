@@ -183,50 +183,227 @@ __global__ __launch_bounds__(256) void k_gemm_strided(StridedGemmDesc d) {
   }
 }
 
-// One subtree of fused steps (steps [s0, s1) in order) by the whole workgroup; leaf(k) gives
-// the pointer of leaf k.
+// Loads and stores through global-memory pointers (address space 1) of any element type, as
+// raw words of its size.
+template <class T>
+struct GlobalRef {
+  using V = std::conditional_t<
+      sizeof(T) == 16, unsigned int __attribute__((ext_vector_type(4))),
+      std::conditional_t<sizeof(T) == 8, unsigned int __attribute__((ext_vector_type(2))),
+                         std::conditional_t<sizeof(T) == 4, unsigned int, unsigned char>>>;
+  static_assert(sizeof(V) == sizeof(T), "element of 1, 4, 8 or 16 bytes");
+  const __attribute__((address_space(1))) V* p;
+  __device__ __forceinline__ T operator[](int i) const { return __builtin_bit_cast(T, p[i]); }
+};
+template <class T>
+__device__ __forceinline__ void store_global(void* base, int i, T v) {
+  using V = typename GlobalRef<T>::V;
+  ((__attribute__((address_space(1))) V*)base)[i] = __builtin_bit_cast(V, v);
+}
+
+// What a wave needs of one step of a chunk: its operand and output locations (global base
+// pointers, or element offsets in the LDS arena as flags say) and its item range.
+struct FusedHead {
+  const void* a;
+  const void* b;
+  void* y;
+  int32_t la, lb, ly;
+  int32_t ibase, nout, nk, flags, pad_;
+};
+// A chunk's step heads and contracted-offset tables, staged in LDS.
+struct FusedStage {
+  FusedHead h[SF_LMAX];
+  int32_t total, lg;
+  uint16_t ka[SF_LMAX][SF_MAXNK], kb[SF_LMAX][SF_MAXNK];
+};
+
+// A value exchanged between lanes (__shfl_xor) as 32-bit words.
+template <class V>
+__device__ __forceinline__ V shfl_xor_words(V v, int mask) {
+  constexpr int NW = (int)((sizeof(V) + 3) / 4);
+  union {
+    V v;
+    int w[NW];
+  } u;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) u.w[i] = 0;
+  u.v = v;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) u.w[i] = __shfl_xor(u.w[i], mask);
+  return u.v;
+}
+
+constexpr int SF_MAXCH = 64;  // chunks of a subtree whose chunk tables are held in LDS
+
+// The LDS a fused subtree works in.
+struct FusedShared {
+  FusedStage stage[2];
+  const void* leaves[SF_MAXLEAF];
+  int32_t tab[SF_MAXCH + 1], itab[SF_MAXCH + 1];
+  __align__(16) char arena[SF_LDS];
+};
+
+// One subtree of fused steps by the whole workgroup: chunks [c0, c1), chunk c being steps
+// [tab[c], tab[c + 1]) with its item table at words [itab[c], itab[c + 1]) of words; leaf(k)
+// gives the pointer of leaf k.  The prologue copies the leaf pointers and the subtree's chunk
+// tables to LDS.  While a chunk runs, the next chunk's step heads (thread j < ns resolves step
+// j's pointers) and contracted-offset tables (one 32-bit word per thread) are loaded, and written
+// to the other stage buffer at the end; each thread works on two items at a time and loads the
+// item-table words of its next two before working on the current ones.  A wave's items belong
+// to one step, found from the staged item bases.
 template <class O, class Leaf>
-__device__ __forceinline__ void fused_subtree(const FusedStep* __restrict__ steps, int s0, int s1,
-                                              Leaf leaf, char* arena, int32_t* ska, int32_t* skb) {
+__device__ __forceinline__ void fused_subtree(const FusedStep* __restrict__ steps,
+                                              const int32_t* __restrict__ gtab,
+                                              const int32_t* __restrict__ gitab,
+                                              const uint32_t* __restrict__ words, int c0, int c1,
+                                              Leaf leaf, char* arena, FusedShared& sh) {
   using T = typename O::T;
-  for (int s = s0; s < s1; ++s) {
-    const FusedStep& d = steps[s];
-    const T* __restrict__ A = d.a_leaf >= 0 ? (const T*)leaf(d.a_leaf) : (const T*)(arena + d.a_off);
-    const T* __restrict__ B = d.b_leaf >= 0 ? (const T*)leaf(d.b_leaf) : (const T*)(arena + d.b_off);
-    T* __restrict__ Y = (T*)(arena + d.y_off);
-    const int nk = d.nk;
-    for (int k = threadIdx.x; k < nk; k += SF_THREADS) {
-      uint32_t r = (uint32_t)k;
-      int32_t oa = 0, ob = 0;
-      for (int q = 0; q < d.ni; ++q) {
-        const uint32_t qq = fdiv_q(r, d.kdiv[q]);
-        const int32_t v = (int32_t)(r - qq * d.kdiv[q].d);
-        oa += v * d.ka[q];
-        ob += v * d.kb[q];
-        r = qq;
-      }
-      ska[k] = oa;
-      skb[k] = ob;
+  using W = AccOps<O>;
+  constexpr int KW = SF_MAXNK / 2;  // 32-bit words of one offset table
+  static_assert(SF_LMAX * 2 * KW <= SF_THREADS && SF_MAXLEAF <= SF_THREADS, "one word per thread");
+  if (c0 >= c1) return;
+  T* L = reinterpret_cast<T*>(sh.arena);
+  const int t = threadIdx.x;
+  const bool held = c1 - c0 <= SF_MAXCH;
+  if (t < SF_MAXLEAF) sh.leaves[t] = leaf(t);
+  if (held && t <= c1 - c0) {
+    sh.tab[t] = gtab[c0 + t];
+    sh.itab[t] = gitab[c0 + t];
+  }
+  __syncthreads();
+  auto tab = [&](int c) { return held ? sh.tab[c - c0] : gtab[c]; };
+  auto itab = [&](int c) { return held ? sh.itab[c - c0] : gitab[c]; };
+  // thread t copies word t % KW of table (t / KW) % 2 of step t / (2 KW)
+  const int cj = t / (2 * KW), ct = (t / KW) & 1, cw = t % KW;
+  uint32_t w = 0;
+  FusedHead hd{};
+  int32_t ctotal = 0, clg = 0;
+  auto fetch = [&](int c) {
+    const int s0 = tab(c), ns = tab(c + 1) - s0;
+    if (cj < ns) {
+      const FusedStep& d = steps[s0 + cj];
+      w = reinterpret_cast<const uint32_t*>(ct ? d.kb : d.ka)[cw];
     }
-    __syncthreads();
-    for (int o = threadIdx.x; o < d.nout; o += SF_THREADS) {
-      uint32_t r = (uint32_t)o;
-      int32_t pa = 0, pb = 0, py = 0;
-      for (int q = 0; q < d.no; ++q) {
-        const uint32_t qq = fdiv_q(r, d.odiv[q]);
-        const int32_t v = (int32_t)(r - qq * d.odiv[q].d);
-        pa += v * d.oa[q];
-        pb += v * d.ob[q];
-        py += v * d.oy[q];
-        r = qq;
+    if (t < ns) {
+      const FusedStep& d = steps[s0 + t];
+      hd.flags = d.flags;
+      hd.ibase = d.ibase;
+      hd.nout = d.nout;
+      hd.nk = d.nk;
+      hd.a = d.a_leaf >= 0 ? sh.leaves[d.a_leaf] : (const void*)(arena + d.a_off);
+      hd.b = d.b_leaf >= 0 ? sh.leaves[d.b_leaf] : (const void*)(arena + d.b_off);
+      hd.y = arena + d.y_off;
+      hd.la = (int32_t)(d.a_off / (int64_t)sizeof(T));
+      hd.lb = (int32_t)(d.b_off / (int64_t)sizeof(T));
+      hd.ly = (int32_t)(d.y_off / (int64_t)sizeof(T));
+      if (t == 0) {
+        ctotal = d.total;
+        clg = d.lg;
       }
-      using W = AccOps<O>;
+    }
+  };
+  auto put = [&](int c, FusedStage& sg) {
+    const int ns = tab(c + 1) - tab(c);
+    if (cj < ns) reinterpret_cast<uint32_t*>(ct ? sg.kb[cj] : sg.ka[cj])[cw] = w;
+    if (t < ns) sg.h[t] = hd;
+    if (t == 0) {
+      sg.total = ctotal;
+      sg.lg = clg;
+    }
+  };
+  auto item = [&](int c, int i) -> uint32_t {
+    const int b = itab(c);
+    return i < itab(c + 1) - b ? words[b + i] : 0u;
+  };
+  fetch(c0);
+  put(c0, sh.stage[0]);
+  uint32_t cur0 = item(c0, t), cur1 = item(c0, t + SF_THREADS);
+  __syncthreads();
+  for (int c = c0; c < c1; ++c) {
+    const FusedStage& sg = sh.stage[(c - c0) & 1];
+    const bool more = c + 1 < c1;
+    if (more) fetch(c + 1);  // in flight while this chunk runs
+    const int ns = tab(c + 1) - tab(c);
+    const int lg = __builtin_amdgcn_readfirstlane(sg.lg), G = 1 << lg;
+    const int items = __builtin_amdgcn_readfirstlane(sg.total);
+    // item wi of the chunk, whose operand offsets are in word
+    auto work = [&](int wi, uint32_t word) {
+      const int wb = __builtin_amdgcn_readfirstlane(wi & ~63);
+      if (wb >= items) return;  // the whole wave
+      int j = 0;
+#pragma unroll
+      for (int u = 1; u < SF_LMAX; ++u) j += (u < ns && wb >= sg.h[u].ibase) ? 1 : 0;
+      j = __builtin_amdgcn_readfirstlane(j);
+      const FusedHead& h = sg.h[j];
+      const int flags = __builtin_amdgcn_readfirstlane(h.flags);
+      const int nk = __builtin_amdgcn_readfirstlane(h.nk);
+      const int rel = wi - h.ibase;
+      const int o = rel >> lg, g = rel & (G - 1);
+      const bool live = o < h.nout;
+      const int ia = (int)(word & 0xffffu), ib = (int)(word >> 16);
       typename W::A::T acc = W::A::zero();
-      for (int k = 0; k < nk; ++k)
-        acc = W::A::fma(W::widen(A[pa + ska[k]]), W::widen(B[pb + skb[k]]), acc);
-      Y[py] = W::narrow(acc);
+      if (live) {
+        // operands in LDS or in global memory (wave-uniform), one code path per pair
+        const T* La = L + h.la + ia;
+        const T* Lb = L + h.lb + ib;
+        // h.a / h.b / h.y are global memory (leaves or the plan's arena)
+        using GV = typename GlobalRef<T>::V;
+        const GlobalRef<T> Ag{(const __attribute__((address_space(1))) GV*)h.a + ia};
+        const GlobalRef<T> Bg{(const __attribute__((address_space(1))) GV*)h.b + ib};
+        const uint16_t* ka = sg.ka[j];
+        const uint16_t* kb = sg.kb[j];
+        auto kloop = [&](auto A, auto B) {
+          int k = g;
+          // four independent operand pairs in flight
+          for (; k + 3 * G < nk; k += 4 * G) {
+            const T a0 = A[ka[k]], a1 = A[ka[k + G]], a2 = A[ka[k + 2 * G]], a3 = A[ka[k + 3 * G]];
+            const T b0 = B[kb[k]], b1 = B[kb[k + G]], b2 = B[kb[k + 2 * G]], b3 = B[kb[k + 3 * G]];
+            acc = W::A::fma(W::widen(a0), W::widen(b0), acc);
+            acc = W::A::fma(W::widen(a1), W::widen(b1), acc);
+            acc = W::A::fma(W::widen(a2), W::widen(b2), acc);
+            acc = W::A::fma(W::widen(a3), W::widen(b3), acc);
+          }
+          for (; k < nk; k += G) acc = W::A::fma(W::widen(A[ka[k]]), W::widen(B[kb[k]]), acc);
+        };
+        if (flags & FS_COPY) {
+          if (g == 0) acc = W::widen(Ag[0]);
+        } else if (flags & FS_A_LDS) {
+          if (flags & FS_B_LDS)
+            kloop(La, Lb);
+          else
+            kloop(La, Bg);
+        } else {
+          if (flags & FS_B_LDS)
+            kloop(Ag, Lb);
+          else
+            kloop(Ag, Bg);
+        }
+      }
+      for (int m = G >> 1; m >= 1; m >>= 1) acc = W::A::add(acc, shfl_xor_words(acc, m));
+      if (live && g == 0) {
+        if (flags & FS_Y_LDS)
+          L[h.ly + o] = W::narrow(acc);
+        else
+          store_global<T>(h.y, o, W::narrow(acc));
+      }
+    };
+    for (int base = 0; base < items; base += 2 * SF_THREADS) {
+      uint32_t n0 = 0, n1 = 0;
+      if (base + 2 * SF_THREADS < items) {
+        n0 = item(c, base + 2 * SF_THREADS + t);
+        n1 = item(c, base + 3 * SF_THREADS + t);
+      } else if (more) {
+        n0 = item(c + 1, t);
+        n1 = item(c + 1, t + SF_THREADS);
+      }
+      work(base + t, cur0);
+      work(base + SF_THREADS + t, cur1);
+      cur0 = n0;
+      cur1 = n1;
     }
-    // this step's outputs are the next steps' operands; the offset table is rewritten next
+    if (!more) break;  // the kernel's end makes the last chunk's outputs visible
+    put(c + 1, sh.stage[(c + 1 - c0) & 1]);  // the buffer chunk c - 1 used
+    // this chunk's outputs are later chunks' operands
     __syncthreads();
   }
 }
@@ -234,25 +411,29 @@ __device__ __forceinline__ void fused_subtree(const FusedStep* __restrict__ step
 template <class O>
 __global__ __launch_bounds__(SF_THREADS) void k_fused_steps(const FusedStep* __restrict__ steps,
                                                             const int32_t* __restrict__ first,
-                                                            FusedLeaves lv, char* arena) {
-  __shared__ int32_t ska[SF_MAXNK], skb[SF_MAXNK];
-  fused_subtree<O>(steps, first[blockIdx.x], first[blockIdx.x + 1],
-                   [&](int k) { return lv.p[k]; }, arena, ska, skb);
+                                                            int nsub, FusedLeaves lv, char* arena) {
+  __shared__ FusedShared sh;
+  const int32_t* tab = first + nsub + 1;
+  const int32_t* itab = tab + first[nsub] + 1;
+  fused_subtree<O>(steps, tab, itab, (const uint32_t*)first, first[blockIdx.x],
+                   first[blockIdx.x + 1], [&](int k) { return lv.p[k]; }, arena, sh);
 }
 
 // Workgroup b runs subtree b - sub_begin of the job whose range holds b.
 template <class O>
 __global__ __launch_bounds__(SF_THREADS) void k_fused_jobs(const FusedJob* __restrict__ jobs,
                                                            int njobs) {
-  __shared__ int32_t ska[SF_MAXNK], skb[SF_MAXNK];
+  __shared__ FusedShared sh;
   const int b = blockIdx.x;
   int j = 0;
   while (j + 1 < njobs && jobs[j + 1].sub_begin <= b) ++j;
   const FusedJob& jb = jobs[j];
   const int q = b - jb.sub_begin;
   const void* const* __restrict__ leaves = jb.leaves;
-  fused_subtree<O>(jb.steps, jb.first[q], jb.first[q + 1], [&](int k) { return leaves[k]; },
-                   jb.arena, ska, skb);
+  const int32_t* tab = jb.first + jb.nsub + 1;
+  const int32_t* itab = tab + jb.first[jb.nsub] + 1;
+  fused_subtree<O>(jb.steps, tab, itab, (const uint32_t*)jb.first, jb.first[q], jb.first[q + 1],
+                   [&](int k) { return leaves[k]; }, jb.arena, sh);
 }
 
 // One thread per output element, K walked directly from global memory: for tiny M/N or K
@@ -630,7 +811,7 @@ void launch_fused_steps(int dt, const FusedStep* steps, const int32_t* first, in
   TimedScope ts(KC_GEMM_VALU, st);
   OME_DISPATCH(dt, O, {
     hipLaunchKernelGGL(k_fused_steps<O>, dim3((uint32_t)nsub), dim3(SF_THREADS), 0, st, steps, first,
-                       leaves, arena);
+                       nsub, leaves, arena);
   });
 }
 

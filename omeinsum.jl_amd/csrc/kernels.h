@@ -100,33 +100,53 @@ struct StridedGemmDesc {
 };
 void launch_gemm_strided(int dt, const StridedGemmDesc& d, hipStream_t st);
 
-// Small tree steps fused into one launch: workgroup b evaluates steps [first[b], first[b+1])
-// of a subtree in order (each the strided contraction above, outputs overwritten), with a
-// workgroup barrier between steps.  Operands are leaves (pointers passed with the launch) or
-// earlier outputs in the plan's arena.
+// Small tree steps fused into one launch: workgroup b evaluates subtree b, each step the
+// strided contraction above with its output overwritten.  A subtree runs as a sequence of
+// chunks, a workgroup barrier after each: a chunk is up to SF_LMAX steps of one level (a step's
+// level is one more than the highest level among its fused operands; steps whose operands are
+// all leaves have level 1), so its steps read only leaves and outputs of earlier chunks.
+// Operands and outputs live in the workgroup's LDS arena (SF_LDS bytes) where they fit: the
+// outputs inside the subtree, and copies of the leaves read above level 1, made by copy steps
+// (y = the leaf) at level 1; the rest are leaves (pointers passed with the launch) or blocks of
+// the plan's arena, as is every subtree root's output.
+// A chunk is a range of work items, each step's from a wave boundary, 2^lg items per output
+// (they split its contracted points).  Item i of a step works on output o = i >> lg (written at
+// element o of y: outputs are enumerated in y's order), whose operand element offsets come from
+// the chunk's item table (a 32-bit word per item: A's offset in the low half, B's in the high).
+// steps holds every subtree's descriptors in chunk order.  first is one int32 array: nsub + 1
+// entries, subtree q's chunks being [first[q], first[q + 1]); then, for nch = first[nsub]
+// chunks, nch + 1 chunk starts (chunk c is descriptors [e[c], e[c + 1])); then nch + 1 item
+// table starts (chunk c's table is the words [f[c], f[c + 1]) of the same array); then the
+// item tables.
 constexpr int SF_MAXO = 16;      // output labels of a fused step
 constexpr int SF_MAXK = 8;       // contracted labels
 constexpr int SF_MAXNK = 64;     // contracted points
 constexpr int SF_MAXOUT = 8192;  // output points
 constexpr int SF_MAXLEAF = 256;  // leaves of a tree whose steps may be fused
 constexpr int SF_THREADS = 1024; // threads of the workgroup that runs one subtree
-struct FusedStep {
-  int32_t a_leaf, b_leaf;        // >= 0: leaf index; < 0: the operand is in the arena
-  int64_t a_off, b_off, y_off;   // arena byte offsets
-  int32_t nout, nk, no, ni;
-  FDiv odiv[SF_MAXO];            // output labels in y's order (first fastest)
-  int32_t oa[SF_MAXO], ob[SF_MAXO], oy[SF_MAXO];
-  FDiv kdiv[SF_MAXK];            // contracted labels
-  int32_t ka[SF_MAXK], kb[SF_MAXK];
+constexpr int SF_LMAX = 16;      // steps of one chunk
+constexpr int SF_LDS = 96 * 1024;  // LDS arena of a subtree, bytes
+enum : int32_t { FS_A_LDS = 1, FS_B_LDS = 2, FS_Y_LDS = 4, FS_COPY = 8 };
+struct alignas(16) FusedStep {
+  int32_t a_leaf, b_leaf;        // >= 0: leaf index, read in place; -1: in the arena or LDS
+  int32_t flags;                 // FS_*: operands / output in the LDS arena; copy step (y = a)
+  int32_t nout, nk;
+  int32_t ibase;                 // first work item of the step in its chunk (a wave boundary)
+  int32_t total, lg;             // the chunk's work items, and log2 of the items per output
+  int64_t a_off, b_off, y_off;   // byte offsets in the plan's arena, or in the LDS arena
+  int64_t pad_;
+  uint16_t ka[SF_MAXNK], kb[SF_MAXNK];  // operand element offsets of each contracted point
 };
+static_assert(sizeof(FusedStep) % 16 == 0, "16-byte descriptors");
 struct FusedLeaves {
   const void* p[SF_MAXLEAF];
 };
 void launch_fused_steps(int dt, const FusedStep* steps, const int32_t* first, int nsub,
                         const FusedLeaves& leaves, char* arena, hipStream_t st);
 // The fused subtrees of several plans (several requests of the request loop) in one launch:
-// job j owns workgroups [sub_begin, sub_begin + nsub) and evaluates its plan's subtrees on its
-// own arena and leaves (a device table of leaf pointers).  jobs is a device array.
+// job j owns workgroups [sub_begin, sub_begin + nsub) and evaluates its plan's subtrees (steps
+// and first as above) on its own arena and leaves (a device table of leaf pointers).  jobs is
+// a device array.
 struct FusedJob {
   const FusedStep* steps;
   const int32_t* first;

@@ -15,6 +15,7 @@
 //   * cost_and_gradient (src/bp.jl:122-160): forward with every node output kept, then the
 //     reverse sweep with einsum_grad (src/autodiff.jl:24-30) using the same planner.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdio>
 #include <functional>
@@ -135,7 +136,7 @@ struct ome_tree {
   // small steps fused into one launch (one workgroup per subtree of small steps)
   std::vector<char> fused;           // per step
   FusedStep* fdev = nullptr;         // device descriptors, subtree by subtree
-  int32_t* ffirst = nullptr;         // device: first descriptor of each subtree (+ end)
+  int32_t* ffirst = nullptr;         // device: subtree, chunk and item tables
   int fsub = 0;                      // subtrees
   // normalised evaluation of Float32 / ComplexF32 trees: the FP64 twin of this plan (same
   // description, Float64 / ComplexF64 elements) and its widened, normalised inputs
@@ -284,60 +285,194 @@ static void plan_fusion(ome_tree* t) {
 // Device descriptors of the fused subtrees (after plan_arena has placed their outputs).
 static Status upload_fusion(ome_tree* t) {
   const int n = (int)t->steps.size();
+  const int64_t es = (int64_t)elem_size(t->dt);
   std::vector<FusedStep> desc;
-  std::vector<int32_t> first;
-  std::function<void(int)> add = [&](int s) {
+  // per descriptor: the operand offsets of each output (A's | B's << 16)
+  std::vector<std::vector<uint32_t>> offs;
+  std::vector<int32_t> first, table;
+  std::vector<int> level(n, 0), members;
+  std::function<int(int)> collect = [&](int s) {  // children first
+    int lv = 0;
     for (auto& a : t->steps[s].args)
-      if (a.ref < 0) add(-(a.ref + 1));
-    const TreeStep& st = t->steps[s];
-    const std::vector<int> x1 = drop_sliced(st.args[0].ix, t->slicing);
-    const std::vector<int> x2 = drop_sliced(st.args[1].ix, t->slicing);
-    const std::vector<int> y = drop_sliced(st.iy, t->slicing);
-    auto stride_in = [&](const std::vector<int>& ls, int l) -> int32_t {
-      int64_t v = 1;
-      for (int l2 : ls) {
-        if (l2 == l) return (int32_t)v;
-        v *= t->sizes_sl[l2];
-      }
-      return 0;
-    };
-    FusedStep d{};
-    const TreeArg& a0 = st.args[0];
-    const TreeArg& a1 = st.args[1];
-    d.a_leaf = a0.ref >= 0 ? a0.ref : -1;
-    d.b_leaf = a1.ref >= 0 ? a1.ref : -1;
-    d.a_off = a0.ref >= 0 ? 0 : t->steps[-(a0.ref + 1)].out_off;
-    d.b_off = a1.ref >= 0 ? 0 : t->steps[-(a1.ref + 1)].out_off;
-    d.y_off = st.out_off;
-    std::vector<int> inner;
-    for (int l : x1)
-      if (has(x2, l) && !has(y, l)) inner.push_back(l);
-    d.nout = (int32_t)prod(y, t->sizes_sl);
-    d.nk = (int32_t)prod(inner, t->sizes_sl);
-    d.no = (int32_t)y.size();
-    d.ni = (int32_t)inner.size();
-    for (int q = 0; q < d.no; ++q) {
-      d.odiv[q] = make_fdiv((uint32_t)t->sizes_sl[y[q]]);
-      d.oa[q] = stride_in(x1, y[q]);
-      d.ob[q] = stride_in(x2, y[q]);
-      d.oy[q] = stride_in(y, y[q]);
-    }
-    for (int q = 0; q < d.ni; ++q) {
-      d.kdiv[q] = make_fdiv((uint32_t)t->sizes_sl[inner[q]]);
-      d.ka[q] = stride_in(x1, inner[q]);
-      d.kb[q] = stride_in(x2, inner[q]);
-    }
-    desc.push_back(d);
+      if (a.ref < 0) lv = std::max(lv, collect(-(a.ref + 1)));
+    members.push_back(s);
+    return level[s] = lv + 1;
   };
-  for (int s = 0; s < n; ++s) {
-    const int parent = t->steps[s].parent;
-    if (!t->fused[s] || (parent >= 0 && t->fused[parent])) continue;  // subtree roots only
-    first.push_back((int32_t)desc.size());
-    add(s);
+  for (int r = 0; r < n; ++r) {
+    const int parent = t->steps[r].parent;
+    if (!t->fused[r] || (parent >= 0 && t->fused[parent])) continue;  // subtree roots only
+    members.clear();
+    collect(r);
+    // the LDS arena: outputs inside the subtree, then the leaves read above level 1, while
+    // they fit (the rest stay in the plan's arena / in place)
+    std::map<int, int64_t> out_lds, leaf_lds;
+    std::map<int, int64_t> leaf_elems;
+    int64_t used = 0;
+    auto fit = [&](int64_t bytes, int64_t* off) {
+      bytes = (bytes + 15) & ~(int64_t)15;
+      if (used + bytes > SF_LDS) return false;
+      *off = used;
+      used += bytes;
+      return true;
+    };
+    for (int s : members) {
+      int64_t off;
+      if (s != r && fit(t->steps[s].out_elems * es, &off)) out_lds[s] = off;
+    }
+    for (int s : members)
+      for (auto& a : t->steps[s].args) {
+        if (a.ref < 0 || level[s] < 2) continue;
+        const int64_t e = prod(drop_sliced(a.ix, t->slicing), t->sizes_sl);
+        int64_t off;
+        if (fit(e * es, &off)) {
+          leaf_lds[a.ref] = off;
+          leaf_elems[a.ref] = e;
+        }
+      }
+    // (level, step, leaf): a step, or the copy of a leaf into the LDS arena at level 1
+    std::vector<std::array<int, 3>> items;
+    for (int s : members) items.push_back({level[s], s, -1});
+    for (auto& kv : leaf_lds) items.push_back({1, -1, kv.first});
+    std::stable_sort(items.begin(), items.end(),
+                     [](const std::array<int, 3>& x, const std::array<int, 3>& y) {
+                       return x[0] < y[0];
+                     });
+    first.push_back((int32_t)table.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+      if (i == 0 || items[i][0] != items[i - 1][0] || (int)desc.size() - table.back() == SF_LMAX)
+        table.push_back((int32_t)desc.size());
+      FusedStep d{};
+      d.a_leaf = d.b_leaf = -1;
+      std::vector<uint32_t> off;
+      if (items[i][2] >= 0) {  // copy step: output o is element o of the leaf
+        const int leaf = items[i][2];
+        d.a_leaf = leaf;
+        d.flags = FS_COPY | FS_Y_LDS;
+        d.y_off = leaf_lds[leaf];
+        d.nout = (int32_t)leaf_elems[leaf];
+        d.nk = 1;
+        for (int o = 0; o < d.nout; ++o) off.push_back((uint32_t)o);
+      } else {
+        const int s = items[i][1];
+        const TreeStep& st = t->steps[s];
+        const std::vector<int> x1 = drop_sliced(st.args[0].ix, t->slicing);
+        const std::vector<int> x2 = drop_sliced(st.args[1].ix, t->slicing);
+        const std::vector<int> y = drop_sliced(st.iy, t->slicing);
+        auto stride_in = [&](const std::vector<int>& ls, int l) -> int64_t {
+          int64_t v = 1;
+          for (int l2 : ls) {
+            if (l2 == l) return v;
+            v *= t->sizes_sl[l2];
+          }
+          return 0;
+        };
+        auto operand = [&](const TreeArg& a, int32_t* leaf, int64_t* aoff, int32_t lds_flag) {
+          if (a.ref >= 0) {
+            auto it = leaf_lds.find(a.ref);
+            if (it != leaf_lds.end()) {
+              d.flags |= lds_flag;
+              *aoff = it->second;
+            } else {
+              *leaf = a.ref;
+            }
+          } else {
+            const int c = -(a.ref + 1);
+            auto it = out_lds.find(c);
+            if (it != out_lds.end()) {
+              d.flags |= lds_flag;
+              *aoff = it->second;
+            } else {
+              *aoff = t->steps[c].out_off;
+            }
+          }
+        };
+        operand(st.args[0], &d.a_leaf, &d.a_off, FS_A_LDS);
+        operand(st.args[1], &d.b_leaf, &d.b_off, FS_B_LDS);
+        auto yo = out_lds.find(s);
+        if (yo != out_lds.end()) {
+          d.flags |= FS_Y_LDS;
+          d.y_off = yo->second;
+        } else {
+          d.y_off = st.out_off;
+        }
+        std::vector<int> inner;
+        for (int l : x1)
+          if (has(x2, l) && !has(y, l)) inner.push_back(l);
+        d.nout = (int32_t)prod(y, t->sizes_sl);
+        d.nk = (int32_t)prod(inner, t->sizes_sl);
+        // operands hold at most nout * nk <= 32768 elements: offsets fit 16 bits
+        auto digits = [&](int64_t idx, const std::vector<int>& ls, int64_t* oa, int64_t* ob) {
+          *oa = *ob = 0;
+          for (int l : ls) {
+            const int64_t v = idx % t->sizes_sl[l];
+            idx /= t->sizes_sl[l];
+            *oa += v * stride_in(x1, l);
+            *ob += v * stride_in(x2, l);
+          }
+        };
+        for (int k = 0; k < d.nk; ++k) {  // contracted point k, first inner label fastest
+          int64_t oa, ob;
+          digits(k, inner, &oa, &ob);
+          if (oa > 65535 || ob > 65535)
+            return Status::err(OME_ERR_UNSUPPORTED, "fused step operand beyond 16-bit offsets");
+          d.ka[k] = (uint16_t)oa;
+          d.kb[k] = (uint16_t)ob;
+        }
+        for (int o = 0; o < d.nout; ++o) {  // output o, first label of y fastest
+          int64_t oa, ob;
+          digits(o, y, &oa, &ob);
+          if (oa > 65535 || ob > 65535)
+            return Status::err(OME_ERR_UNSUPPORTED, "fused step operand beyond 16-bit offsets");
+          off.push_back((uint32_t)oa | ((uint32_t)ob << 16));
+        }
+      }
+      desc.push_back(d);
+      offs.push_back(std::move(off));
+    }
   }
-  first.push_back((int32_t)desc.size());
+  first.push_back((int32_t)table.size());
+  table.push_back((int32_t)desc.size());
   t->fsub = (int)first.size() - 1;
   if (t->fsub <= 0) return Status::ok();
+  // per chunk: the work items per output (2^lg lanes splitting its contracted points: a power
+  // of two, at most 16 and at most the chunk's largest contracted extent, while the workgroup
+  // holds all the items), each step's items from a wave boundary so that a wave works on one
+  // step; then the chunk's item table
+  const int nch = (int)table.size() - 1;
+  const size_t head = first.size() + table.size() + (size_t)nch + 1;
+  std::vector<int32_t> ibases;
+  std::vector<uint32_t> words;
+  auto items_of = [](int64_t nout, int lg) { return ((nout << lg) + 63) & ~(int64_t)63; };
+  for (int c = 0; c < nch; ++c) {
+    int32_t maxnk = 1;
+    for (int i = table[c]; i < table[c + 1]; ++i) maxnk = std::max(maxnk, desc[i].nk);
+    auto count = [&](int lg) {
+      int64_t m = 0;
+      for (int i = table[c]; i < table[c + 1]; ++i) m += items_of(desc[i].nout, lg);
+      return m;
+    };
+    int lg = 0;
+    while (lg < 4 && (2 << lg) <= maxnk && count(lg + 1) <= SF_THREADS) ++lg;
+    ibases.push_back((int32_t)(head + words.size()));
+    int32_t ibase = 0;
+    for (int i = table[c]; i < table[c + 1]; ++i) {
+      desc[i].ibase = ibase;
+      desc[i].lg = lg;
+      const int64_t m = items_of(desc[i].nout, lg);
+      for (int64_t it = 0; it < m; ++it) {
+        const int64_t o = it >> lg;
+        words.push_back(o < desc[i].nout ? offs[i][o] : 0u);
+      }
+      ibase += (int32_t)m;
+    }
+    for (int i = table[c]; i < table[c + 1]; ++i) desc[i].total = ibase;
+  }
+  ibases.push_back((int32_t)(head + words.size()));
+  if (head + words.size() > (size_t)INT32_MAX)
+    return Status::err(OME_ERR_UNSUPPORTED, "fused step tables beyond 2^31 words");
+  first.insert(first.end(), table.begin(), table.end());
+  first.insert(first.end(), ibases.begin(), ibases.end());
+  for (uint32_t w : words) first.push_back((int32_t)w);
   ome_ctx* ctx = t->ctx;
   t->fdev = (FusedStep*)ctx->pool->alloc(desc.size() * sizeof(FusedStep));
   t->ffirst = (int32_t*)ctx->pool->alloc(first.size() * sizeof(int32_t));
