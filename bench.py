@@ -733,7 +733,7 @@ def main():
     if args.workload in ("all", "c1"):
         extra["c1_ijk_jkl_il_64"] = bench_c1(args.steps, args.warmup)
     if args.workload in ("all", "c2"):
-        extra["c2_network_100"] = bench_c2(max(5, args.steps // 2), 2)
+        extra["c2_network_100"] = bench_c2(max(5, args.steps // 2), 20)
     if args.workload in ("all", "c3"):
         extra["c3_grouped"] = bench_c3(max(3, args.steps // 4), 1, not args.no_per_problem)
     if args.workload in ("all", "c5"):

@@ -357,8 +357,9 @@ struct BitPermDesc {
 
 __device__ __forceinline__ int bp_swz(int i) { return i ^ (((i >> 5) ^ (i >> 10)) & 31); }
 
+// One tile (workgroup-local index b of the permute's 2^rbits tiles) of the permute d.
 template <class O>
-__global__ __launch_bounds__(256) void k_permute_bits(BitPermDesc d) {
+__device__ __forceinline__ void permute_bits_tile(const BitPermDesc& d, uint32_t b) {
   using T = typename O::T;
   constexpr int CAP = sizeof(T) > 8 ? 2048 : 4096;
   __shared__ T tile[CAP];
@@ -366,7 +367,6 @@ __global__ __launch_bounds__(256) void k_permute_bits(BitPermDesc d) {
   T* __restrict__ y = (T*)d.y;
   const T sx = O::from(d.sx), sy = O::from(d.sy);
   const int t = threadIdx.x;
-  uint32_t b = blockIdx.x;
   int64_t ib = 0, ob = 0;
   for (int r = 0; r < d.rbits; ++r)
     if ((b >> r) & 1u) { ib += d.r_in[r]; ob += d.r_out[r]; }
@@ -391,6 +391,21 @@ __global__ __launch_bounds__(256) void k_permute_bits(BitPermDesc d) {
       const T e = conj_if<O>(tile[bp_swz(tl + d.k_lds[k])], d.conj);
       y[o] = epilogue<O>(d.mode, sx, sy, e, y + o);
     }
+}
+
+template <class O>
+__global__ __launch_bounds__(256) void k_permute_bits(BitPermDesc d) {
+  permute_bits_tile<O>(d, blockIdx.x);
+}
+
+// Two permutes in one launch: workgroups [0, 2^d0.rbits) move d0's tiles, the rest d1's.
+template <class O>
+__global__ __launch_bounds__(256) void k_permute_bits2(BitPermDesc d0, BitPermDesc d1) {
+  const uint32_t n0 = 1u << d0.rbits;
+  if (blockIdx.x < n0)
+    permute_bits_tile<O>(d0, blockIdx.x);
+  else
+    permute_bits_tile<O>(d1, blockIdx.x - n0);
 }
 
 // The bit-permutation form of d, or false when some extent is not a power of two or the
@@ -473,19 +488,39 @@ static bool bit_perm_desc(int dt, const PermDesc& d, BitPermDesc& bp) {
   return true;
 }
 
+static bool permute_bits_on() {
+  static int on = -1;
+  if (on < 0) {
+    const char* e = getenv("OME_PERMUTE_BITS");
+    on = (e && e[0] == '0') ? 0 : 1;
+  }
+  return on == 1;
+}
+
+void launch_permute2(int dt, const PermDesc& d0, const PermDesc& d1, hipStream_t st) {
+  BitPermDesc b0, b1;
+  if (d0.n > 0 && d1.n > 0 && permute_bits_on() && bit_perm_desc(dt, d0, b0) &&
+      bit_perm_desc(dt, d1, b1) && b0.rbits < 31 && b1.rbits < 31) {
+    OME_DRY_RETURN(1);
+    TimedScope ts(KC_PERMUTE, st);
+    const uint32_t blocks = (1u << b0.rbits) + (1u << b1.rbits);
+    OME_DISPATCH(dt, O, {
+      hipLaunchKernelGGL(k_permute_bits2<O>, dim3(blocks), dim3(256), 0, st, b0, b1);
+    });
+    return;
+  }
+  launch_permute(dt, d0, st);
+  launch_permute(dt, d1, st);
+}
+
 void launch_permute(int dt, const PermDesc& d, hipStream_t st) {
   if (d.n <= 0) return;
   OME_DRY_RETURN(1);
   TimedScope ts(KC_PERMUTE, st);
   // power-of-two extents (all of a bond-dimension-2 network's tensors): the bit-permutation
   // tile.  OME_PERMUTE_BITS=0 turns it off for measurements.
-  static int bits_on = -1;
-  if (bits_on < 0) {
-    const char* e = getenv("OME_PERMUTE_BITS");
-    bits_on = (e && e[0] == '0') ? 0 : 1;
-  }
   BitPermDesc bp;
-  if (bits_on && bit_perm_desc(dt, d, bp)) {
+  if (permute_bits_on() && bit_perm_desc(dt, d, bp)) {
     const uint32_t blocks = 1u << bp.rbits;
     OME_DISPATCH(dt, O, {
       hipLaunchKernelGGL(k_permute_bits<O>, dim3(blocks), dim3(256), 0, st, bp);

@@ -73,6 +73,9 @@ void launch_normalize(int dt, void* x, int64_t n, double* work, double* logacc, 
 void launch_widen_normalized(int dt, const void* x, void* y, int64_t n, double* work,
                              double* logacc, hipStream_t st);
 void launch_permute(int dt, const PermDesc& d, hipStream_t st);
+// Two independent permutes (a GEMM's two operands); one launch when both take the
+// bit-permutation path, else one after the other.
+void launch_permute2(int dt, const PermDesc& d0, const PermDesc& d1, hipStream_t st);
 void launch_walk(int dt, const WalkDesc& d, hipStream_t st);
 
 int64_t gemm_valu_workspace(int dt, const GemmDesc& d, int ksplit);

@@ -54,9 +54,12 @@ static std::string fmt_labels(const std::vector<int>& v) {
 // ---- permute -----------------------------------------------------------------------------
 
 // y[labels oy] <- sy*y + sx*conj?(x[labels ix]) for a permutation oy of the unique labels ix.
+// With `out`, a permute that needs the permute kernel is not launched: its descriptor is
+// written to *out (and *deferred set) for the caller to launch.
 static Status do_permute(int dt, const void* x, const std::vector<int>& ix, void* y,
                          const std::vector<int>& oy, const std::vector<int64_t>& sizes,
-                         const Scal& sx, const Scal& sy, int conj, hipStream_t st) {
+                         const Scal& sx, const Scal& sy, int conj, hipStream_t st,
+                         PermDesc* out = nullptr, bool* deferred = nullptr) {
   const int r = (int)ix.size();
   int64_t n = prod_sizes(ix, sizes);
   if (n == 0) return Status::ok();
@@ -109,9 +112,17 @@ static Status do_permute(int dt, const void* x, const std::vector<int>& ix, void
   d.sy = sy;
   OME_DISPATCH(dt, O, { d.mode = epi_mode<O>(sx, sy); });
   d.conj = conj;
+  if (out) {
+    *out = d;
+    *deferred = true;
+    return Status::ok();
+  }
   launch_permute(dt, d, st);
   return Status::ok();
 }
+
+// Whether y[iy] <- x[ix] is a plain permutation (unique labels, the same set).
+static bool is_permutation(const std::vector<int>& ix, const std::vector<int>& iy);
 
 // ---- index walk ----------------------------------------------------------------------------
 
@@ -198,6 +209,15 @@ static Status do_walk(ome_ctx* ctx, int dt, const std::vector<const void*>& xs,
 
 // ---- unary -----------------------------------------------------------------------------------
 
+static bool is_permutation(const std::vector<int>& ix, const std::vector<int>& iy) {
+  std::vector<int> ux = uniq(ix), uy = uniq(iy);
+  bool perm = ux.size() == ix.size() && uy.size() == iy.size() && ux.size() == uy.size();
+  if (perm)
+    for (int l : iy)
+      if (!contains(ix, l)) perm = false;
+  return perm;
+}
+
 static Status do_unary(ome_ctx* ctx, int dt, const void* x, const std::vector<int>& ix, void* y,
                        const std::vector<int>& iy, const std::vector<int64_t>& sizes,
                        const Scal& sx, const Scal& sy, int conj, hipStream_t st,
@@ -206,12 +226,7 @@ static Status do_unary(ome_ctx* ctx, int dt, const void* x, const std::vector<in
     launch_axpby(dt, x, y, prod_sizes(iy, sizes), sx, sy, conj, st);
     return Status::ok();
   }
-  std::vector<int> ux = uniq(ix), uy = uniq(iy);
-  bool perm = ux.size() == ix.size() && uy.size() == iy.size() && ux.size() == uy.size();
-  if (perm)
-    for (int l : iy)
-      if (!contains(ix, l)) perm = false;
-  if (perm) return do_permute(dt, x, ix, y, iy, sizes, sx, sy, conj, st);
+  if (is_permutation(ix, iy)) return do_permute(dt, x, ix, y, iy, sizes, sx, sy, conj, st);
   const void* src = x;
   if (conj) {
     keep.emplace_back(ctx, (size_t)prod_sizes(ix, sizes) * elem_size(dt));
@@ -575,27 +590,40 @@ static Status do_binary(ome_ctx* ctx, const EinsumCall& c, const std::vector<int
     for (auto* p : parts) v.insert(v.end(), p->begin(), p->end());
     return v;
   };
-  // materialise A as [o1, in, ba] / B as [in, o2, ba] when not read in place
+  // materialise A as [o1, in, ba] / B as [in, o2, ba] when not read in place; when both are
+  // plain permutes needing the permute kernel, they go out as one launch
   const void* pa = x1;
   std::vector<int> la = ix1;
+  const void* pb = x2;
+  std::vector<int> lb = ix2;
+  if (!best.a_inplace) la = cat({&L.o1, &L.in, &L.ba});
+  if (!best.b_inplace) lb = cat({&L.in, &L.o2, &L.ba});
+  const bool pair = !best.a_inplace && !best.b_inplace && is_permutation(ix1, la) &&
+                    is_permutation(ix2, lb);
+  PermDesc pda{}, pdb{};
+  bool dfa = false, dfb = false;
   if (!best.a_inplace) {
-    la = cat({&L.o1, &L.in, &L.ba});
     keep.emplace_back(ctx, (size_t)na * elem_size(dt));
     if (!keep.back().p && na) return Status::err(OME_ERR_OOM, "device pool exhausted");
-    Status s = do_unary(ctx, dt, x1, ix1, keep.back().p, la, sz, one, zero, 0, st, keep);
+    Status s = pair ? do_permute(dt, x1, ix1, keep.back().p, la, sz, one, zero, 0, st, &pda, &dfa)
+                    : do_unary(ctx, dt, x1, ix1, keep.back().p, la, sz, one, zero, 0, st, keep);
     if (!s.good()) return s;
     pa = keep.back().p;
   }
-  const void* pb = x2;
-  std::vector<int> lb = ix2;
   if (!best.b_inplace) {
-    lb = cat({&L.in, &L.o2, &L.ba});
     keep.emplace_back(ctx, (size_t)nb * elem_size(dt));
     if (!keep.back().p && nb) return Status::err(OME_ERR_OOM, "device pool exhausted");
-    Status s = do_unary(ctx, dt, x2, ix2, keep.back().p, lb, sz, one, zero, 0, st, keep);
+    Status s = pair ? do_permute(dt, x2, ix2, keep.back().p, lb, sz, one, zero, 0, st, &pdb, &dfb)
+                    : do_unary(ctx, dt, x2, ix2, keep.back().p, lb, sz, one, zero, 0, st, keep);
     if (!s.good()) return s;
     pb = keep.back().p;
   }
+  if (dfa && dfb)
+    launch_permute2(dt, pda, pdb, st);
+  else if (dfa)
+    launch_permute(dt, pda, st);
+  else if (dfb)
+    launch_permute(dt, pdb, st);
   void* py = c.y;
   std::vector<int> ly = iy;
   Scal gsx = c.sx, gsy = c.sy;
