@@ -283,6 +283,7 @@ def bench_c2(steps, warmup):
     for name, be, ins, out_y in (
             ("mi355x_fp32", om.MI355XBackend(precision="fp32"), dxs, y),
             ("mi355x_bf16x3", om.MI355XBackend(precision="bf16x3"), dxs, y),
+            ("mi355x_bf16x6", om.MI355XBackend(precision="bf16x6"), dxs, y),
             ("mi355x_fp64_precision", om.MI355XBackend(precision="fp64"), dxs, y),
             ("mi355x_complexf64", om.MI355XBackend(), dxs128, y128),
             ("per_step_fp32", om.DefaultBackend(), dxs, y)):
