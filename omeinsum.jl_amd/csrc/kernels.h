@@ -121,8 +121,6 @@ void launch_gemm_strided(int dt, const StridedGemmDesc& d, hipStream_t st);
 // chunks, nch + 1 chunk starts (chunk c is descriptors [e[c], e[c + 1])); then nch + 1 item
 // table starts (chunk c's table is the words [f[c], f[c + 1]) of the same array); then the
 // item tables.
-constexpr int SF_MAXO = 16;      // output labels of a fused step
-constexpr int SF_MAXK = 8;       // contracted labels
 constexpr int SF_MAXNK = 64;     // contracted points
 constexpr int SF_MAXOUT = 8192;  // output points
 constexpr int SF_MAXLEAF = 256;  // leaves of a tree whose steps may be fused

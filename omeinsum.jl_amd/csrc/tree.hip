@@ -268,8 +268,8 @@ static void plan_fusion(ome_tree* t) {
       if (!has(x1, l) && !has(x2, l)) cover = false;
     if (!cover) continue;
     const int64_t nout = prod(y, t->sizes_sl), nk = prod(inner, t->sizes_sl);
-    ok[s] = nout <= SF_MAXOUT && nk <= SF_MAXNK && nout * nk <= (1 << 15) &&
-            (int)y.size() <= SF_MAXO && (int)inner.size() <= SF_MAXK;
+    // (operands then hold at most 2^15 elements: 16-bit offsets in the item tables)
+    ok[s] = nout <= SF_MAXOUT && nk <= SF_MAXNK && nout * nk <= (1 << 15);
   }
   int count = 0;
   for (int s = 0; s < n; ++s) {  // post-order: children first
