@@ -217,20 +217,17 @@ struct FusedStage {
   uint16_t ka[SF_LMAX][SF_MAXNK], kb[SF_LMAX][SF_MAXNK];
 };
 
-// A value exchanged between lanes (__shfl_xor) as 32-bit words.
+// Shuffle of any trivially copyable value across the lanes of a wave, word by word.
 template <class V>
-__device__ __forceinline__ V shfl_xor_words(V v, int mask) {
+__device__ __forceinline__ V shfl_xor_any(V v, int mask) {
   constexpr int NW = (int)((sizeof(V) + 3) / 4);
-  union {
-    V v;
-    int w[NW];
-  } u;
+  uint32_t w[NW] = {};
+  __builtin_memcpy(w, &v, sizeof(V));
 #pragma unroll
-  for (int i = 0; i < NW; ++i) u.w[i] = 0;
-  u.v = v;
-#pragma unroll
-  for (int i = 0; i < NW; ++i) u.w[i] = __shfl_xor(u.w[i], mask);
-  return u.v;
+  for (int i = 0; i < NW; ++i) w[i] = (uint32_t)__shfl_xor((int)w[i], mask, 64);
+  V r;
+  __builtin_memcpy(&r, w, sizeof(V));
+  return r;
 }
 
 constexpr int SF_MAXCH = 64;  // chunks of a subtree whose chunk tables are held in LDS
@@ -379,7 +376,7 @@ __device__ __forceinline__ void fused_subtree(const FusedStep* __restrict__ step
             kloop(Ag, Bg);
         }
       }
-      for (int m = G >> 1; m >= 1; m >>= 1) acc = W::A::add(acc, shfl_xor_words(acc, m));
+      for (int m = G >> 1; m >= 1; m >>= 1) acc = W::A::add(acc, shfl_xor_any(acc, m));
       if (live && g == 0) {
         if (flags & FS_Y_LDS)
           L[h.ly + o] = W::narrow(acc);
@@ -553,18 +550,6 @@ __global__ __launch_bounds__(256) void k_dot_reduce_wg(GemmDesc d) {
   *yp = epilogue<O>(d.mode, O::from(d.sx), O::from(d.sy), W::narrow(red[0]), yp);
 }
 
-// Shuffle of any trivially copyable value across the lanes of a wave, word by word.
-template <class V>
-__device__ __forceinline__ V shfl_xor_any(V v, int mask) {
-  constexpr int NW = (int)((sizeof(V) + 3) / 4);
-  uint32_t w[NW] = {};
-  __builtin_memcpy(w, &v, sizeof(V));
-#pragma unroll
-  for (int i = 0; i < NW; ++i) w[i] = (uint32_t)__shfl_xor((int)w[i], mask, 64);
-  V r;
-  __builtin_memcpy(&r, w, sizeof(V));
-  return r;
-}
 
 // Few outputs, long K, in one launch: workgroup b owns a TM x TN output tile over all of K.
 // Its PN_THREADS threads stride K, each summing the tile's TM*TN products in the
